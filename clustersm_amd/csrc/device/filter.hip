@@ -1,0 +1,275 @@
+// Candidate filtering on gfx950.
+//
+// nlc_filter: one wave per block of 64 consecutive data vertices (lane =
+// vertex).  Low-degree vertices count their neighbour labels serially into a
+// per-lane LDS histogram over the query's label slots; vertices above
+// kLightDegree are counted by the whole wave with LDS atomics.  The
+// predicate of filter.cu:486-507 is then evaluated per query vertex and the
+// 64 answers become one bitmap word through a ballot.
+//
+// refine_kernel: the cluster refinement, a "pull" test: for each set bit v of
+// the bitmap being refined, is there a neighbour of v in every one of the
+// given support bitmaps?  Early exit on the first hit.
+#include <algorithm>
+#include <chrono>
+
+#include "common.hpp"
+
+namespace csm {
+
+namespace {
+
+constexpr int kFilterThreads = 256;
+constexpr int kFilterWaves = kFilterThreads / kWave;
+constexpr uint32_t kLightDegree = 64;
+constexpr uint32_t kLabelTable = 1024;
+constexpr uint8_t kNoSlot = 0xFF;
+
+struct FilterSmem {
+  QueryDesc d;
+  uint8_t slot_of[kLabelTable];
+  uint8_t hist[kFilterWaves][CSM_MAX_QV][kWave];
+  uint32_t heavy[kFilterWaves][CSM_MAX_QV];
+  unsigned long long cnt[CSM_MAX_QV];
+};
+
+__device__ inline uint32_t lookup_slot(const FilterSmem &sm, uint32_t l) {
+  if (l < kLabelTable) return sm.slot_of[l];
+  for (uint32_t s = 0; s < sm.d.num_slots; ++s)
+    if (sm.d.slot_label[s] == l) return s;
+  return kNoSlot;
+}
+
+__global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, const uint32_t *__restrict__ label,
+    uint32_t n, const QueryDesc *__restrict__ gd, uint64_t *__restrict__ bits,
+    unsigned long long *__restrict__ counts, unsigned long long *__restrict__ scanned_deg) {
+  __shared__ FilterSmem sm;
+  const int tid = threadIdx.x;
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(gd);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.d);
+    for (uint32_t i = tid; i < sizeof(QueryDesc) / 4; i += kFilterThreads) dst[i] = src[i];
+    for (uint32_t i = tid; i < kLabelTable; i += kFilterThreads) sm.slot_of[i] = kNoSlot;
+    if (tid < CSM_MAX_QV) sm.cnt[tid] = 0;
+  }
+  __syncthreads();
+  if (tid < static_cast<int>(sm.d.num_slots) && sm.d.slot_label[tid] < kLabelTable)
+    sm.slot_of[sm.d.slot_label[tid]] = static_cast<uint8_t>(tid);
+  __syncthreads();
+
+  const int wave = tid / kWave, lane = tid % kWave;
+  const uint32_t words = sm.d.words, nq = sm.d.nq, nslots = sm.d.num_slots;
+  uint64_t scanned = 0;
+  for (uint32_t b = blockIdx.x * kFilterWaves + wave; b < words; b += gridDim.x * kFilterWaves) {
+    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
+    uint32_t s = kNoSlot, dv = 0, o0 = 0;
+    if (v < n) {
+      s = lookup_slot(sm, label[v]);
+      o0 = off[v];
+      dv = off[v + 1] - o0;
+    }
+    const bool active = s != kNoSlot && dv >= sm.d.slot_min_deg[s];
+    for (uint32_t sl = 0; sl < nslots; ++sl) sm.hist[wave][sl][lane] = 0;
+    if (active) scanned += dv;
+    if (active && dv <= kLightDegree) {
+      for (uint32_t o = o0; o < o0 + dv; ++o) {
+        const uint32_t s2 = lookup_slot(sm, label[adj[o]]);
+        if (s2 != kNoSlot && sm.hist[wave][s2][lane] < 255) sm.hist[wave][s2][lane]++;
+      }
+    }
+    uint64_t heavy = ballot(active && dv > kLightDegree);
+    while (heavy) {
+      const int h = __builtin_ctzll(heavy);
+      heavy &= heavy - 1;
+      const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
+      if (lane < CSM_MAX_QV) sm.heavy[wave][lane] = 0;
+      wave_sync();
+      for (uint32_t o = ho + lane; o < ho + hd; o += kWave) {
+        const uint32_t s2 = lookup_slot(sm, label[adj[o]]);
+        if (s2 != kNoSlot) atomicAdd(&sm.heavy[wave][s2], 1u);
+      }
+      wave_sync();
+      if (lane < static_cast<int>(nslots)) sm.hist[wave][lane][h] = static_cast<uint8_t>(min(sm.heavy[wave][lane], 255u));
+      wave_sync();
+    }
+    wave_sync();
+    for (uint32_t u = 0; u < nq; ++u) {
+      bool ok = active && sm.d.slot[u] == s && sm.d.deg[u] <= dv;
+      if (ok)
+        for (uint32_t t = sm.d.sig_off[u]; t < sm.d.sig_off[u + 1]; ++t)
+          if (sm.hist[wave][sm.d.sig_slot[t]][lane] < sm.d.sig_cnt[t]) {
+            ok = false;
+            break;
+          }
+      const uint64_t m = ballot(ok);
+      if (lane == 0) {
+        bits[static_cast<uint64_t>(u) * words + b] = m;
+        if (m) atomicAdd(&sm.cnt[u], static_cast<unsigned long long>(__popcll(m)));
+      }
+    }
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) scanned += __shfl_xor(scanned, d, kWave);
+  if (lane == 0 && scanned) atomicAdd(scanned_deg, static_cast<unsigned long long>(scanned));
+  __syncthreads();
+  if (tid < static_cast<int>(nq) && sm.cnt[tid]) atomicAdd(&counts[tid], sm.cnt[tid]);
+}
+
+// CORE: x = cluster core, supports = the members (all must be adjacent).
+// MEMBER: x = member blockIdx.y, support = the core.
+template <bool CORE>
+__global__ __launch_bounds__(kFilterThreads) void refine_kernel(
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n, uint32_t words,
+    uint64_t *__restrict__ bits, const uint32_t *__restrict__ cl_us, uint32_t cb, uint32_t ce,
+    unsigned long long *__restrict__ scanned_deg) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint32_t x = CORE ? cl_us[cb] : cl_us[cb + 1 + blockIdx.y];
+  const uint32_t y0 = CORE ? cb + 1 : cb, y1 = CORE ? ce : cb + 1;
+  uint64_t *xrow = bits + static_cast<uint64_t>(x) * words;
+  uint64_t scanned = 0;
+  for (uint32_t b = blockIdx.x * kFilterWaves + wave; b < words; b += gridDim.x * kFilterWaves) {
+    const uint64_t xw = xrow[b];
+    if (xw == 0) continue;
+    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
+    const bool act = (xw >> lane) & 1ull;
+    uint32_t o0 = 0, dv = 0;
+    if (act && v < n) {
+      o0 = off[v];
+      dv = off[v + 1] - o0;
+    }
+    bool ok = act;
+    if (act && dv <= kLightDegree) {
+      for (uint32_t yi = y0; yi < y1 && ok; ++yi) {
+        const uint64_t *yrow = bits + static_cast<uint64_t>(cl_us[yi]) * words;
+        bool found = false;
+        for (uint32_t o = o0; o < o0 + dv && !found; ++o) {
+          const uint32_t w = adj[o];
+          found = (yrow[w >> 6] >> (w & 63)) & 1ull;
+          scanned++;
+        }
+        ok = found;
+      }
+    }
+    uint64_t heavy = ballot(act && dv > kLightDegree);
+    while (heavy) {
+      const int h = __builtin_ctzll(heavy);
+      heavy &= heavy - 1;
+      const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
+      bool hok = true;
+      for (uint32_t yi = y0; yi < y1 && hok; ++yi) {
+        const uint64_t *yrow = bits + static_cast<uint64_t>(cl_us[yi]) * words;
+        bool any = false;
+        for (uint32_t o = ho; o < ho + hd && !any; o += kWave) {
+          bool f = false;
+          if (o + lane < ho + hd) {
+            const uint32_t w = adj[o + lane];
+            f = (yrow[w >> 6] >> (w & 63)) & 1ull;
+            scanned++;
+          }
+          any = ballot(f) != 0;
+        }
+        hok = any;
+      }
+      if (lane == h) ok = hok;
+    }
+    const uint64_t nw = ballot(ok);
+    if (lane == 0 && nw != xw) xrow[b] = nw;
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) scanned += __shfl_xor(scanned, d, kWave);
+  if (lane == 0 && scanned) atomicAdd(scanned_deg, static_cast<unsigned long long>(scanned));
+}
+
+__global__ __launch_bounds__(kFilterThreads) void count_bits_kernel(const uint64_t *__restrict__ bits, uint32_t words,
+                                                                    unsigned long long *__restrict__ counts) {
+  const uint32_t u = blockIdx.y;
+  unsigned long long c = 0;
+  for (uint32_t b = blockIdx.x * kFilterThreads + threadIdx.x; b < words; b += gridDim.x * kFilterThreads)
+    c += __popcll(bits[static_cast<uint64_t>(u) * words + b]);
+  for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(&counts[u], c);
+}
+
+uint32_t grid_words(uint32_t words, uint32_t per_block) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(1u << 20, ceil_div(words, per_block)));
+}
+
+}  // namespace
+
+// Filter + refinement for one query on ctx->stream; fills q->counts.
+csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine) {
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx->stream;
+  const QueryDesc &d = q->desc;
+  const uint32_t words = d.words, nq = d.nq;
+  unsigned long long *dev = static_cast<unsigned long long *>(ctx->arena.alloc(8 * (2 * CSM_MAX_QV + 4)));
+  if (!dev) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (filter counters)");
+  unsigned long long *counts = dev, *scan_filter = dev + CSM_MAX_QV, *scan_refine = dev + CSM_MAX_QV + 1;
+  CSM_HIP_TRY(ctx, hipMemsetAsync(dev, 0, 8 * (2 * CSM_MAX_QV + 4), s));
+  std::memset(q->kstat, 0, sizeof(q->kstat));
+  KernelTimer kt(ctx, q->kstat);
+  kt.begin(CSM_K_FILTER);
+  nlc_filter_kernel<<<grid_words(words, kFilterWaves), kFilterThreads, 0, s>>>(g->off, g->adj, g->label, g->n,
+                                                                              q->d_desc, q->bits, counts,
+                                                                              scan_filter);
+  // B_f = labels + offsets of every vertex + (neighbour id + neighbour label)
+  // for every scanned vertex + the bitmap words; the scanned part is added
+  // below once the device counter is read.
+  kt.end(CSM_K_FILTER, 4ull * g->n + 4ull * (g->n + 1ull) + 8ull * words * nq);
+  const csm_query_plan &p = q->plan;
+  if (refine) {
+    for (uint32_t c = 0; c < p.num_clusters; ++c) {
+      const uint32_t cb = p.cluster_offsets[c], ce = p.cluster_offsets[c + 1];
+      if (ce - cb < 2) continue;
+      kt.begin(CSM_K_REFINE);
+      refine_kernel<true><<<grid_words(words, kFilterWaves), kFilterThreads, 0, s>>>(
+          g->off, g->adj, g->n, words, q->bits, q->d_cluster_us, cb, ce, scan_refine);
+      kt.end(CSM_K_REFINE, 8ull * words * (ce - cb));
+      kt.begin(CSM_K_REFINE);
+      refine_kernel<false><<<dim3(grid_words(words, kFilterWaves), ce - cb - 1), kFilterThreads, 0, s>>>(
+          g->off, g->adj, g->n, words, q->bits, q->d_cluster_us, cb, ce, scan_refine);
+      kt.end(CSM_K_REFINE, 16ull * words * (ce - cb - 1));
+    }
+    CSM_HIP_TRY(ctx, hipMemsetAsync(counts, 0, 8 * CSM_MAX_QV, s));
+    kt.begin(CSM_K_AUX);
+    count_bits_kernel<<<dim3(grid_words(words, kFilterThreads), nq), kFilterThreads, 0, s>>>(q->bits, words, counts);
+    kt.end(CSM_K_AUX, 8ull * words * nq);
+  }
+  unsigned long long host[2 * CSM_MAX_QV + 4];
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(host, dev, sizeof(host), hipMemcpyDeviceToHost, s));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  kt.flush();
+  q->kstat[CSM_K_FILTER].bytes += 8ull * host[CSM_MAX_QV];
+  q->kstat[CSM_K_REFINE].bytes += 12ull * host[CSM_MAX_QV + 1];  // adjacency word + bitmap word per probe
+  for (uint32_t u = 0; u < nq; ++u) q->counts[u] = host[u];
+  ctx->arena.free(dev);
+  q->filtered = true;
+  q->ms_filter = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return CSM_OK;
+}
+
+}  // namespace csm
+
+using namespace csm;
+
+extern "C" {
+
+csm_status csm_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine, uint64_t *counts) {
+  if (!ctx || !g || !q || q->graph_n != g->n) return CSM_E_ARG;
+  hipSetDevice(ctx->device);
+  csm_status st = run_filter(ctx, g, q, refine);
+  if (st != CSM_OK) return st;
+  if (counts)
+    for (uint32_t u = 0; u < q->desc.nq; ++u) counts[u] = q->counts[u];
+  return CSM_OK;
+}
+
+csm_status csm_candidates_download(csm_ctx *ctx, const csm_query *q, uint64_t *bits) {
+  if (!ctx || !q || !bits) return CSM_E_ARG;
+  hipSetDevice(ctx->device);
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(bits, q->bits, 8ull * q->desc.words * q->desc.nq, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return CSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+"""GPU parity of the plain path (device CSR build, filter, refinement, join)
+against the CPU oracle and the committed goldens.  Bit-exact: counts, sorted
+embedding lists, candidate bitmaps."""
+import random
+
+import numpy as np
+import pytest
+
+import clustersm_amd as csm
+from helpers import brute_force, example_paths, list_digest, load_example, load_golden, rand_graph, rand_query
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = csm.Context(0, 8 << 30)
+    yield c
+    c.close()
+
+
+def _host_csr(n, edges):
+    nbr = [set() for _ in range(n)]
+    for a, b in edges:
+        if a != b:
+            nbr[a].add(b)
+            nbr[b].add(a)
+    off = np.zeros(n + 1, np.uint64)
+    adj = []
+    for v in range(n):
+        s = sorted(nbr[v])
+        adj += s
+        off[v + 1] = off[v] + len(s)
+    return off, np.array(adj, np.uint32)
+
+
+def test_device_csr_build(ctx):
+    r = random.Random(1)
+    for it in range(20):
+        n = r.randint(1, 300)
+        m = r.randint(0, 3 * n)
+        edges = [(r.randrange(n), r.randrange(n)) for _ in range(m)]  # loops and duplicates
+        edges += edges[: m // 4]
+        labels = [r.randrange(5) for _ in range(n)]
+        print("csr case", it, n, len(edges), flush=True)
+        g = ctx.upload_edges(labels, edges if edges else np.zeros((0, 2), np.uint32))
+        off, adj, lab = g.download()
+        eo, ea = _host_csr(n, edges)
+        assert (off.astype(np.uint64) == eo).all(), it
+        assert (adj == ea).all(), it
+        assert (lab == np.array(labels, np.uint32)).all()
+        nv, ne, md = g.info()
+        assert ne * 2 == len(ea) and md == (int(np.diff(eo).max()) if n else 0)
+        g.free()
+
+
+def test_example_end_to_end(ctx):
+    qp, gp = example_paths()
+    ex = load_example()
+    g = ctx.upload(csm.HostGraph.read(gp))
+    q = g.prepare(csm.HostGraph.read(qp).plan())
+    counts = q.filter(refine=False)
+    assert counts.tolist() == [len(s) for s in ex["nlc_candidates"]]
+    bits = q.candidates()
+    assert [list(map(int, s)) for s in O.bits_to_sets(bits, 11)] == ex["nlc_candidates"]
+    q.filter(refine=True)
+    assert [list(map(int, s)) for s in O.bits_to_sets(q.candidates(), 11)] == ex["refined_candidates"]
+    r, rows = q.match(want_list=True)
+    assert r.count == 4 and not r.capped
+    assert [tuple(x) for x in rows.tolist()] == [tuple(e) for e in ex["embeddings"]]
+    r, _ = q.match()
+    assert r.count == 4
+    # levels 2..4 materialised: 3 + 3 + 7 rows (the non-tree check is fused into the last level)
+    assert r.partial_rows == 13
+    q.free()
+    g.free()
+
+
+@pytest.mark.parametrize("case", load_golden(), ids=lambda c: c["name"])
+def test_golden_cases(ctx, case):
+    g = ctx.upload_edges(case["data_labels"], case["data_edges"])
+    q = g.prepare(csm.HostGraph.from_edges(case["query_labels"], case["query_edges"]).plan())
+    for refine in (False, True):
+        r, rows = q.match(want_list=True, list_capacity=1 << 16, refine=refine)
+        assert r.count == case["count"] and not r.capped
+        assert list_digest([tuple(x) for x in rows.tolist()]) == case["sha256"]
+        r, _ = q.match(refine=refine)
+        assert r.count == case["count"]
+    q.free()
+    g.free()
+
+
+def _oracle_all(dl, de, ql, qe, cap=(1 << 64) - 1, list_cap=1 << 20):
+    d = O.Data.from_arrays(dl, de)
+    oq = O.Query.from_arrays(ql, qe)
+    op = oq.plan()
+    return d, oq, op, O.match(d, oq, op, cap=cap, want_list=True, list_cap=list_cap)
+
+
+def test_filter_and_refine_bitmaps_vs_oracle(ctx):
+    r = random.Random(2)
+    for it in range(25):
+        n = r.randint(50, 3000)
+        nl = r.randint(1, 6)
+        dl, de = rand_graph(r, min(n, 200), r.uniform(0.02, 0.2), nl) if it % 2 else (None, None)
+        if dl is None:
+            hg = csm.gen_erdos_renyi(n, min(4 * n, n * (n - 1) // 2), nl, it)
+            dl, de = hg.arrays()
+            dl, de = dl.tolist(), de.tolist()
+        nv = len(dl)
+        nq = r.randint(2, 10)
+        ql, qe = rand_query(r, nq, r.randint(0, nq), nl)
+        print("bitmap case", it, nv, len(de), nq, flush=True)
+        d = O.Data.from_arrays(dl, de)
+        oq = O.Query.from_arrays(ql, qe)
+        op = oq.plan()
+        g = ctx.upload_edges(dl, de)
+        q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+        q.filter(refine=False)
+        assert (q.candidates() == O.filter_nlc(d, nv, oq)).all(), it
+        q.filter(refine=True)
+        assert (q.candidates() == O.refine(d, nv, oq, op)).all(), it
+        q.free()
+        g.free()
+
+
+def test_random_matches_vs_oracle_with_small_chunks(ctx):
+    """Many chunks per level (DFS across chunks) must not change the result or its order."""
+    r = random.Random(4)
+    for it in range(40):
+        n = r.randint(20, 120)
+        nl = r.randint(1, 3)
+        dl, de = rand_graph(r, n, r.uniform(0.05, 0.2), nl)
+        nq = r.randint(3, 6)
+        ql, qe = rand_query(r, nq, r.randint(0, 3), nl)
+        d, oq, op, (oc, ocap, orows, _, _) = _oracle_all(dl, de, ql, qe, list_cap=200000)
+        if oc > 200000:
+            continue
+        print("match case", it, n, len(de), nq, oc, flush=True)
+        g = ctx.upload_edges(dl, de)
+        q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+        maxdeg = g.info()[2]
+        for chunk in (0, max(maxdeg, 256)):
+            res, rows = q.match(want_list=True, list_capacity=200000, chunk_items=chunk)
+            assert res.count == oc, (it, chunk)
+            assert np.array_equal(rows, np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, nq))
+            res, _ = q.match(chunk_items=chunk)
+            assert res.count == oc
+        q.free()
+        g.free()
+
+
+def test_caps_match_oracle(ctx):
+    r = random.Random(6)
+    done = 0
+    for it in range(60):
+        dl, de = rand_graph(r, 40, 0.25, 2)
+        ql, qe = rand_query(r, 4, r.randint(0, 2), 2)
+        d = O.Data.from_arrays(dl, de)
+        oq = O.Query.from_arrays(ql, qe)
+        op = oq.plan()
+        total = O.match(d, oq, op)[0]
+        if total < 10:
+            continue
+        g = ctx.upload_edges(dl, de)
+        q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+        for cap in (1, total // 3, total - 1, total, total + 5):
+            oc, ocap, orows, _, _ = O.match(d, oq, op, cap=cap, want_list=True)
+            res, rows = q.match(cap=cap, want_list=True, chunk_items=256)
+            assert (res.count, bool(res.capped)) == (oc, ocap), (it, cap)
+            assert np.array_equal(rows, np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, 4))
+            res, _ = q.match(cap=cap)
+            assert (res.count, bool(res.capped)) == (oc, ocap)
+        q.free()
+        g.free()
+        done += 1
+    assert done > 5
+
+
+def test_config1_er_full_list(ctx):
+    """ER 1000/4000 (seed 1), 4 uniform labels, 4-vertex random-walk query (seed 2), full list."""
+    hg = csm.gen_erdos_renyi(1000, 4000, 4, 1)
+    hq = csm.gen_query(hg, 4, csm.QUERY_INDUCED, 2)
+    dl, de = hg.arrays()
+    ql, qe = hq.arrays()
+    d, oq, op, (oc, _, orows, _, _) = _oracle_all(dl, de, ql, qe)
+    g = ctx.upload(hg)
+    q = g.prepare(hq.plan())
+    res, rows = q.match(want_list=True, list_capacity=max(oc, 1))
+    print("config1 count", res.count, flush=True)
+    assert res.count == oc
+    assert np.array_equal(rows, np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, 4))
+    q.free()
+    g.free()
+
+
+def test_config2_rmat16_counts(ctx):
+    """R-MAT scale 16, 16 Zipf labels: two 8-vertex queries, tree-like and with 3 extra cycle edges."""
+    hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    off, adj, lab = g.download()
+    d = O.Data.from_arrays(lab, host.arrays()[1])
+    for mode, extra, seed in ((csm.QUERY_TREE, 0, 11), (csm.QUERY_EXTRA, 3, 12)):
+        hq = csm.gen_query(host, 8, mode, seed, extra_edges=extra)
+        ql, qe = hq.arrays()
+        oq = O.Query.from_arrays(ql, qe)
+        op = oq.plan()
+        cap = 2_000_000
+        oc, ocap, _, _, _ = O.match(d, oq, op, cap=cap, threads=8)
+        q = g.prepare(hq.plan())
+        res, _ = q.match(cap=cap)
+        print("config2", mode, res.count, bool(res.capped), flush=True)
+        assert (res.count, bool(res.capped)) == (oc, ocap)
+        q.free()
+    g.free()
