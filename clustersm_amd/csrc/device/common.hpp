@@ -68,6 +68,12 @@ struct csm_graph {
   uint32_t *adj = nullptr; // 2m
   uint32_t *label = nullptr;
   uint32_t max_label = 0;
+  // Dense adjacency bitmaps of the highest-degree vertices: an edge probe
+  // against a hub is one load instead of a binary search.
+  int32_t *hub_id = nullptr;     // n entries, -1 for non-hubs
+  uint64_t *hub_bits = nullptr;  // num_hubs x ceil(n / 64)
+  uint32_t num_hubs = 0;
+  uint32_t hub_min_degree = 0;
 };
 
 namespace csm {
@@ -90,6 +96,8 @@ struct QueryDesc {
 };
 
 // Matching order as the join kernels read it (one level = one position).
+// The anchors of a position are its parent followed by its non-tree
+// neighbours: a new vertex must be adjacent to the vertices of every anchor.
 struct LevelDesc {
   uint32_t u;            // query vertex matched at this position
   uint32_t parent;       // parent position
@@ -97,6 +105,29 @@ struct LevelDesc {
   uint32_t back[CSM_MAX_QV];
   uint32_t ninj;         // earlier positions with the same label (injectivity)
   uint32_t inj[CSM_MAX_QV];
+  uint32_t nanchor;      // 1 + nback
+  uint32_t anchor[CSM_MAX_QV + 1];
+};
+
+// Edge probe: dense bitmap for hubs, binary search in the sorted list otherwise.
+struct AdjProbe {
+  const uint32_t *off;
+  const uint32_t *adj;
+  const int32_t *hub_id;
+  const uint64_t *hub_bits;
+  uint32_t words;
+  __device__ inline bool has(uint32_t x, uint32_t w) const {
+    const int32_t h = hub_id[x];
+    if (h >= 0) return (hub_bits[static_cast<uint64_t>(h) * words + (w >> 6)] >> (w & 63)) & 1ull;
+    uint32_t lo = off[x], hi = off[x + 1];
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      const uint32_t y = adj[mid];
+      if (y == w) return true;
+      if (y < w) lo = mid + 1; else hi = mid;
+    }
+    return false;
+  }
 };
 
 }  // namespace csm

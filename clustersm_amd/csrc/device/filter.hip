@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kFilterThreads) void count_bits_kernel(const uint64
 }
 
 uint32_t grid_words(uint32_t words, uint32_t per_block) {
-  return std::max<uint32_t>(1, std::min<uint32_t>(1u << 20, ceil_div(words, per_block)));
+  return std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(words, per_block)));
 }
 
 }  // namespace

@@ -3,20 +3,28 @@
 // Partial embeddings are materialised level by level, as in the reference
 // (join.cu:290-435), but in bounded chunks: "BFS within a chunk, DFS across
 // chunks".  Level k holds rows of k vertex ids (matching-order positions
-// 0..k-1, row-major).  For a chunk of rows, every (row, neighbour of the
-// row's parent) pair is one item; items are numbered through the exclusive
-// prefix sum of the parents' degrees, so a workgroup tile of 256 consecutive
-// items covers a contiguous run of rows (each row has at least one item).
+// 0..k-1, row-major).
 //
-//   expand<MARK>   test each item, write a 64-bit survivor mask per wave
-//   scan           tile survivor counts -> output offsets
-//   materialize    write surviving rows (prefix + new vertex) in item order
-//   expand<COUNT>  last level: count survivors only
+// Extending a row to position k needs a vertex adjacent to the row's vertex
+// at every anchor of k (its tree parent and its non-tree neighbours).  Each
+// row enumerates the shortest of those adjacency lists (the "driver") and
+// probes the others (hub bitmap or binary search).  Every (row, driver
+// neighbour) pair is one item; items are numbered through the exclusive
+// prefix sum of the drivers' degrees, so a workgroup tile of 256 consecutive
+// items covers a contiguous run of rows.
 //
-// Item order is (row, ascending neighbour), so with ascending root
-// candidates every level is in lexicographic order of the matching-order
-// tuple, and DFS over chunks enumerates embeddings in that order: a cap cuts
-// the same embeddings as the CPU backtracking oracle.
+//   expand<MARK>        test each item, 64-bit survivor mask per wave
+//   scan                tile survivor counts -> output offsets
+//   expand<MATERIALIZE> write surviving rows (prefix + new vertex), item order
+//   expand<COUNT>       last position: count survivors
+//   expand<FUSED>       last position without non-tree edges, counted from the
+//                       position before it: each surviving item adds
+//                       |N(x) ∩ C(u_last)| (precomputed per vertex x) minus
+//                       the row's own vertices in that set
+//
+// Item order is (row, ascending new vertex): every level is in lexicographic
+// order of the matching-order tuple and DFS over chunks enumerates in that
+// order, so a cap cuts the same embeddings as the CPU backtracking oracle.
 #include <algorithm>
 #include <chrono>
 #include <numeric>
@@ -31,26 +39,32 @@ namespace {
 
 constexpr int kTile = 256;          // items per workgroup
 constexpr int kTileWaves = kTile / kWave;
+constexpr uint32_t kLightDegree = 64;
 
 struct ExpandArgs {
-  const uint32_t *off;
-  const uint32_t *adj;
+  AdjProbe probe;
   const uint32_t *rows;     // level k, stride k
-  const uint64_t *dscan;    // level k parent-degree prefix, n_k + 1 entries
+  const uint64_t *dscan;    // level k driver-degree prefix, n_k + 1 entries
+  const uint8_t *drv;       // driver anchor index per row
+  const uint32_t *tile_row; // first row of each tile, relative to row_a (ntiles + 1 entries)
   const uint64_t *cand;     // candidate bitmap row of the new vertex
   uint64_t row_a, row_b;    // chunk rows [a, b)
   uint64_t item_base;       // dscan[a]
   uint64_t items;           // dscan[b] - dscan[a]
-  uint32_t k;               // columns of the input rows
-  LevelDesc lv;             // position k: parent, non-tree checks, injectivity
+  uint32_t k;               // columns of the input rows = position being matched
+  LevelDesc lv;             // position k
   uint64_t *mask;           // MARK: items / 64 words
   uint64_t *tile_cnt;       // MARK: survivors per tile
   const uint64_t *tile_off; // MATERIALIZE: exclusive prefix of tile_cnt
   uint32_t *out;            // MATERIALIZE: rows of k + 1 ids
-  unsigned long long *count;  // COUNT
+  unsigned long long *count;  // COUNT / FUSED
+  // FUSED: the last position (k + 1)
+  LevelDesc lv_last;
+  const uint64_t *cand_last;
+  const uint32_t *cnt;      // |N(x) ∩ C(u_last)| per vertex x
 };
 
-enum { kMark = 0, kCount = 1, kMaterialize = 2 };
+enum { kMark = 0, kCount = 1, kMaterialize = 2, kFused = 3 };
 
 // Largest r in [lo, hi) with dscan[r] <= x (dscan[lo] <= x < dscan[hi]).
 __device__ inline uint64_t upper_row(const uint64_t *dscan, uint64_t lo, uint64_t hi, uint64_t x) {
@@ -61,59 +75,86 @@ __device__ inline uint64_t upper_row(const uint64_t *dscan, uint64_t lo, uint64_
   return lo;
 }
 
-__device__ inline bool adjacent(const uint32_t *off, const uint32_t *adj, uint32_t x, uint32_t w) {
-  uint32_t lo = off[x], hi = off[x + 1];
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    uint32_t y = adj[mid];
-    if (y == w) return true;
-    if (y < w) lo = mid + 1; else hi = mid;
-  }
-  return false;
+__device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
+
+template <typename T>
+__device__ inline T block_sum(T v, T *lds) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[wave] = v;
+  __syncthreads();
+  T t = 0;
+  for (int x = 0; x < kTileWaves; ++x) t += lds[x];
+  return t;
 }
 
+constexpr int kTileRows = kTile + 2;  // rows a tile can touch (each row has >= 1 item)
+
+// Dynamic LDS: the tile's rows (kTileRows x k ids).
 template <int MODE>
 __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
-  __shared__ uint64_t sd[kTile + 1];
-  __shared__ uint64_t s_rlo;
+  extern __shared__ uint32_t s_rows[];
+  __shared__ uint64_t sd[kTileRows + 1];
+  __shared__ uint32_t s_base[kTileRows];
   __shared__ uint32_t s_wcnt[kTileWaves];
+  __shared__ unsigned long long s_sum[kTileWaves];
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kTile;
-  if (tid == 0) s_rlo = upper_row(a.dscan, a.row_a, a.row_b, a.item_base + first);
+  const uint64_t rlo = a.row_a + a.tile_row[blockIdx.x];
+  const uint64_t rend = min(a.row_a + a.tile_row[blockIdx.x + 1] + 1, a.row_b);  // exclusive
+  const uint32_t nr = static_cast<uint32_t>(rend - rlo);
+  const uint32_t k = a.k;
+  // stage: driver-degree prefix, rows (coalesced), driver list offset per row
+  for (uint32_t i = tid; i <= nr; i += kTile) sd[i] = a.dscan[rlo + i];
+  for (uint32_t i = tid; i < nr * k; i += kTile) s_rows[i] = a.rows[rlo * k + i];
   __syncthreads();
-  const uint64_t rlo = s_rlo;
-  const uint64_t nwin = min(static_cast<uint64_t>(kTile + 1), a.row_b + 1 - rlo);
-  for (uint64_t i = tid; i < nwin; i += kTile) sd[i] = a.dscan[rlo + i];
+  for (uint32_t i = tid; i < nr; i += kTile) {
+    const uint32_t xd = s_rows[i * k + a.lv.anchor[a.drv[rlo + i]]];
+    s_base[i] = a.probe.off[xd];
+  }
   __syncthreads();
 
   const uint64_t item = first + tid;
   const bool valid = item < a.items;
   bool ok = false;
   uint32_t w = 0;
-  uint64_t r = 0;
+  uint32_t ri = 0;
+  unsigned long long contrib = 0;
   if (valid) {
     const uint64_t gi = a.item_base + item;
-    uint64_t lo = 0, hi = nwin - 1;  // sd[lo] <= gi < sd[hi] within the window
-    if (gi >= sd[nwin - 1]) {        // only with zero-degree rows: global search
-      r = upper_row(a.dscan, rlo + nwin - 1, a.row_b, gi);
-    } else {
-      while (hi - lo > 1) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (sd[mid] <= gi) lo = mid; else hi = mid;
-      }
-      r = rlo + lo;
+    uint32_t lo = 0, hi = nr;  // sd[lo] <= gi < sd[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sd[mid] <= gi) lo = mid; else hi = mid;
     }
-    const uint32_t *row = a.rows + r * a.k;
-    const uint64_t rs = (r - rlo < nwin) ? sd[r - rlo] : a.dscan[r];
-    const uint32_t p = row[a.lv.parent];
-    w = a.adj[a.off[p] + static_cast<uint32_t>(gi - rs)];
+    ri = lo;
+    const uint32_t *row = s_rows + ri * k;
+    w = a.probe.adj[s_base[ri] + static_cast<uint32_t>(gi - sd[ri])];
     if (MODE == kMaterialize) {
       ok = (a.mask[item >> 6] >> (item & 63)) & 1ull;
     } else {
-      ok = (a.cand[w >> 6] >> (w & 63)) & 1ull;
+      const uint32_t d = a.drv[rlo + ri];
+      ok = bit_of(a.cand, w);
       for (uint32_t t = 0; ok && t < a.lv.ninj; ++t) ok = row[a.lv.inj[t]] != w;
-      for (uint32_t t = 0; ok && t < a.lv.nback; ++t) ok = adjacent(a.off, a.adj, row[a.lv.back[t]], w);
+      for (uint32_t t = 0; ok && t < a.lv.nanchor; ++t)
+        if (t != d) ok = a.probe.has(row[a.lv.anchor[t]], w);
+      if (MODE == kFused && ok) {
+        const uint32_t pk = a.lv_last.parent;
+        const uint32_t x = pk == k ? w : row[pk];
+        long long c = a.cnt[x];
+        for (uint32_t t = 0; t < a.lv_last.ninj; ++t) {
+          const uint32_t j = a.lv_last.inj[t];
+          const uint32_t v = j == k ? w : row[j];
+          if (v != x && bit_of(a.cand_last, v) && a.probe.has(x, v)) --c;
+        }
+        contrib = static_cast<unsigned long long>(c);
+      }
     }
+  }
+  if (MODE == kFused) {  // per-tile partial; one reduction kernel sums them
+    const unsigned long long tot = block_sum(contrib, s_sum);
+    if (tid == 0) a.tile_cnt[blockIdx.x] = tot;
+    return;
   }
   const uint64_t m = ballot(ok);
   if (MODE == kMaterialize) {
@@ -122,10 +163,10 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
     if (ok) {
       uint64_t pos = a.tile_off[blockIdx.x] + lanes_below(m);
       for (int x = 0; x < wave; ++x) pos += s_wcnt[x];
-      const uint32_t *row = a.rows + r * a.k;
-      uint32_t *dst = a.out + pos * (a.k + 1);
-      for (uint32_t j = 0; j < a.k; ++j) dst[j] = row[j];
-      dst[a.k] = w;
+      const uint32_t *row = s_rows + ri * k;
+      uint32_t *dst = a.out + pos * (k + 1);
+      for (uint32_t j = 0; j < k; ++j) dst[j] = row[j];
+      dst[k] = w;
     }
     return;
   }
@@ -137,22 +178,111 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   if (tid == 0) {
     uint64_t c = 0;
     for (int x = 0; x < kTileWaves; ++x) c += s_wcnt[x];
-    if (MODE == kMark) a.tile_cnt[blockIdx.x] = c;
-    if (MODE == kCount && c) atomicAdd(a.count, static_cast<unsigned long long>(c));
+    a.tile_cnt[blockIdx.x] = c;
   }
 }
 
-// dscan[i] = degree of the parent vertex of row i; dscan[n] = 0.
-__global__ void parent_degree_kernel(const uint32_t *off, const uint32_t *rows, uint32_t k, uint32_t parent,
-                                     uint64_t n, uint64_t *dscan) {
+// *out += sum of vals[0..n) (one workgroup; avoids same-address atomics per tile).
+__global__ __launch_bounds__(1024) void reduce_add_kernel(const uint64_t *vals, uint64_t n, unsigned long long *out) {
+  __shared__ unsigned long long lds[1024 / kWave];
+  unsigned long long acc = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += 1024) acc += vals[i];
+  for (int d = kWave / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < 1024 / kWave; ++w) t += lds[w];
+    *out += t;
+  }
+}
+
+// tile_row[t] = row (relative to a) holding item t * kTile of the chunk
+// [a, b); tile_row[ntiles] = b - a.
+__global__ void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, uint64_t ntiles, uint32_t *tile_row) {
+  const uint64_t base = dscan[a];
+  for (uint64_t r = a + blockIdx.x * 256ull + threadIdx.x; r < b; r += 256ull * gridDim.x) {
+    const uint64_t lo = dscan[r] - base, hi = dscan[r + 1] - base;  // items [lo, hi)
+    for (uint64_t t = (lo + kTile - 1) / kTile; t * kTile < hi; ++t) tile_row[t] = static_cast<uint32_t>(r - a);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) tile_row[ntiles] = static_cast<uint32_t>(b - a);
+}
+
+// dscan[i] = smallest degree among the row's anchor vertices, drv[i] = that
+// anchor's index; dscan[n] = 0.
+__global__ void anchor_degree_kernel(const uint32_t *off, const uint32_t *rows, uint32_t k, LevelDesc lv, uint64_t n,
+                                     uint64_t *dscan, uint8_t *drv) {
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i <= n; i += 256ull * gridDim.x) {
     if (i == n) {
       dscan[i] = 0;
-    } else {
-      uint32_t p = rows[i * k + parent];
-      dscan[i] = off[p + 1] - off[p];
+      continue;
     }
+    const uint32_t *row = rows + i * k;
+    uint32_t best = 0xFFFFFFFFu, bi = 0;
+    for (uint32_t t = 0; t < lv.nanchor; ++t) {
+      const uint32_t x = row[lv.anchor[t]];
+      const uint32_t dg = off[x + 1] - off[x];
+      if (dg < best) {
+        best = dg;
+        bi = t;
+      }
+    }
+    dscan[i] = best;
+    drv[i] = static_cast<uint8_t>(bi);
   }
+}
+
+// cnt[v] = |N(v) ∩ C(u_last)| for v in C(u_parent), 0 elsewhere.
+__global__ __launch_bounds__(kTile) void cand_nbr_count_kernel(const uint32_t *__restrict__ off,
+                                                               const uint32_t *__restrict__ adj, uint32_t n,
+                                                               uint32_t words, const uint64_t *__restrict__ xbits,
+                                                               const uint64_t *__restrict__ ybits,
+                                                               uint32_t *__restrict__ cnt) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (uint32_t b = blockIdx.x * kTileWaves + wave; b < words; b += gridDim.x * kTileWaves) {
+    const uint64_t xw = xbits[b];
+    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
+    const bool act = (xw >> lane) & 1ull;
+    uint32_t o0 = 0, dv = 0, c = 0;
+    if (act) {
+      o0 = off[v];
+      dv = off[v + 1] - o0;
+    }
+    if (act && dv <= kLightDegree)
+      for (uint32_t o = o0; o < o0 + dv; ++o) c += bit_of(ybits, adj[o]);
+    uint64_t heavy = ballot(act && dv > kLightDegree);
+    while (heavy) {
+      const int h = __builtin_ctzll(heavy);
+      heavy &= heavy - 1;
+      const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
+      uint32_t part = 0;
+      for (uint32_t o = ho + lane; o < ho + hd; o += kWave) part += bit_of(ybits, adj[o]);
+      for (int s = kWave / 2; s > 0; s >>= 1) part += __shfl_xor(part, s, kWave);
+      if (lane == h) c = part;
+    }
+    if (v < n) cnt[v] = c;
+  }
+}
+
+// Rows of the last level counted directly (queries of two vertices).
+__global__ __launch_bounds__(kTile) void count_rows_kernel(AdjProbe probe, const uint32_t *rows, uint32_t k,
+                                                           uint64_t n, LevelDesc lv_last,
+                                                           const uint64_t *cand_last, const uint32_t *cnt,
+                                                           unsigned long long *count) {
+  __shared__ unsigned long long s_sum[kTileWaves];
+  unsigned long long acc = 0;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+    const uint32_t *row = rows + i * k;
+    const uint32_t x = row[lv_last.parent];
+    long long c = cnt[x];
+    for (uint32_t t = 0; t < lv_last.ninj; ++t) {
+      const uint32_t v = row[lv_last.inj[t]];
+      if (v != x && bit_of(cand_last, v) && probe.has(x, v)) --c;
+    }
+    acc += static_cast<unsigned long long>(c);
+  }
+  const unsigned long long tot = block_sum(acc, s_sum);
+  if (threadIdx.x == 0 && tot) atomicAdd(count, tot);
 }
 
 __global__ void word_popcount_kernel(const uint64_t *bits, uint32_t words, uint64_t *cnt) {
@@ -195,9 +325,9 @@ uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(
 
 // Control block slots (device and pinned host mirror)
 enum : int {
-  kCtlCount = 0,     // survivors counted at the last level
+  kCtlCount = 0,     // embeddings counted at the last position
   kCtlTotal = 1,     // scan total (rows materialised by the last step)
-  kCtlChunkA = 2,    // chunk choice for the new level (b, items, base)
+  kCtlChunkA = 2,    // chunk choice for a new level (b, items, base)
   kCtlChunkB = 5,    // chunk choice for the current level (b, items, base)
   kCtlSize = 8
 };
@@ -207,8 +337,9 @@ struct Workspace {
   std::vector<void *> blocks;
   uint32_t *rows[CSM_MAX_QV + 1] = {};
   uint64_t *dscan[CSM_MAX_QV + 1] = {};
+  uint8_t *drv[CSM_MAX_QV + 1] = {};
   uint64_t *mask = nullptr, *tiles = nullptr, *partials = nullptr, *ctl = nullptr;
-  uint64_t item_cap = 0;
+  uint32_t *cnt = nullptr, *tile_row = nullptr;
   ~Workspace() {
     for (void *p : blocks) ctx->arena.free(p);
   }
@@ -218,15 +349,6 @@ struct Workspace {
     return p;
   }
 };
-
-// Per-item bytes of the workspace across all levels.
-uint64_t bytes_per_item(uint32_t nq, bool list) {
-  uint64_t b = 0;
-  for (uint32_t k = 2; k < nq; ++k) b += 4ull * k + 8;  // rows of k ids + degree prefix
-  if (list) b += 4ull * nq;
-  b += 1 + 8 / 256 + 1;  // mask bit, tile counter, scan partials (rounded up)
-  return b;
-}
 
 }  // namespace
 
@@ -238,49 +360,76 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   const bool want_list = o.list != nullptr;
   const uint64_t cap = o.cap ? o.cap : ~0ull;
   const LevelDesc *lv = q->level;
+  const uint32_t L = nq - 1;  // last position
   KernelTimer kt(ctx, res.kernels);
+  AdjProbe probe{g->off, g->adj, g->hub_id, g->hub_bits, words};
+
+  // How the last position is counted.
+  const bool precount = !want_list && lv[L].nback == 0;
+  const bool fused = precount && L >= 2;     // counted from level L-1 items
+  const bool rows2 = precount && L == 1;     // two-vertex query: count level-1 rows
+  const uint32_t stop = fused ? L - 1 : L;   // deepest level whose items are expanded
 
   const uint64_t n1 = q->counts[lv[0].u];
   res.count = 0;
   res.capped = 0;
   res.listed = 0;
-  if (n1 == 0) {
+  auto finish_time = [&]() {
     res.ms_join = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  if (n1 == 0) {
+    finish_time();
     return CSM_OK;
   }
   Workspace ws;
   ws.ctx = ctx;
   // chunk size: requested, else what fits in half of the largest free block
-  uint64_t item_cap = o.chunk_items ? o.chunk_items : (64ull << 20);
-  const uint64_t per_item = bytes_per_item(nq, want_list);
-  const uint64_t fixed = 8ull * (n1 + 1) + 4ull * n1 + 8ull * (words + 1) + (1 << 20);
+  uint64_t per_item = 16;  // mask bit, tile counter, scan partials, rounded up
+  for (uint32_t k = 2; k <= stop; ++k) per_item += 4ull * k + 8 + 1;
+  if (want_list) per_item += 4ull * nq;
+  const uint64_t fixed = 13ull * (n1 + 1) + 8ull * (words + 1) + (precount ? 4ull * g->n : 0) + (1 << 20);
   const uint64_t avail = ctx->arena.largest_free();
   if (avail < fixed + per_item * g->max_degree * 2)
     return set_error(ctx, CSM_E_NOMEM, "arena too small for the join workspace");
+  uint64_t item_cap = o.chunk_items ? o.chunk_items : (64ull << 20);
   item_cap = std::min(item_cap, (avail - fixed) / 2 / per_item);
   item_cap = std::max<uint64_t>(item_cap, g->max_degree);
   item_cap = (item_cap + kTile - 1) / kTile * kTile;
-  ws.item_cap = item_cap;
   const uint64_t max_tiles = item_cap / kTile + 2;
   const uint64_t scan_n = std::max<uint64_t>({item_cap + 1, n1 + 1, words + 1ull, max_tiles});
   ws.ctl = static_cast<uint64_t *>(ws.get(8 * kCtlSize));
   ws.mask = static_cast<uint64_t *>(ws.get(8 * (item_cap / 64 + kTileWaves)));
   ws.tiles = static_cast<uint64_t *>(ws.get(8 * max_tiles));
+  ws.tile_row = static_cast<uint32_t *>(ws.get(4 * (max_tiles + 1)));
   ws.partials = static_cast<uint64_t *>(ws.get(8 * (scan_n / kScanTile + 2)));
   ws.rows[1] = static_cast<uint32_t *>(ws.get(4 * n1));
   ws.dscan[1] = static_cast<uint64_t *>(ws.get(8 * std::max<uint64_t>(n1 + 1, words + 1ull)));
-  bool okalloc = ws.ctl && ws.mask && ws.tiles && ws.partials && ws.rows[1] && ws.dscan[1];
-  for (uint32_t k = 2; k < nq && okalloc; ++k) {
+  ws.drv[1] = static_cast<uint8_t *>(ws.get(n1 + 1));
+  bool okalloc = ws.ctl && ws.mask && ws.tiles && ws.tile_row && ws.partials && ws.rows[1] && ws.dscan[1] && ws.drv[1];
+  for (uint32_t k = 2; k <= stop && okalloc; ++k) {
     ws.rows[k] = static_cast<uint32_t *>(ws.get(4ull * k * item_cap));
     ws.dscan[k] = static_cast<uint64_t *>(ws.get(8ull * (item_cap + 1)));
-    okalloc = ws.rows[k] && ws.dscan[k];
+    ws.drv[k] = static_cast<uint8_t *>(ws.get(item_cap + 1));
+    okalloc = ws.rows[k] && ws.dscan[k] && ws.drv[k];
   }
   if (okalloc && want_list) {
     ws.rows[nq] = static_cast<uint32_t *>(ws.get(4ull * nq * item_cap));
     okalloc = ws.rows[nq] != nullptr;
   }
+  if (okalloc && precount) {
+    ws.cnt = static_cast<uint32_t *>(ws.get(4ull * g->n));
+    okalloc = ws.cnt != nullptr;
+  }
   if (!okalloc) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (join workspace)");
   CSM_HIP_TRY(ctx, hipMemsetAsync(ws.ctl, 0, 8 * kCtlSize, s));
+  const uint64_t *cand_last = q->bits + static_cast<uint64_t>(lv[L].u) * words;
+  if (precount) {
+    const uint64_t *xbits = q->bits + static_cast<uint64_t>(lv[lv[L].parent].u) * words;
+    kt.begin(CSM_K_AUX);
+    cand_nbr_count_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(words, kTileWaves))), kTile, 0,
+                            s>>>(g->off, g->adj, g->n, words, xbits, cand_last, ws.cnt);
+    kt.end(CSM_K_AUX, 8ull * words + 4ull * g->n + 12ull * q->counts[lv[lv[L].parent].u]);
+  }
 
   // level 1: root candidates, ascending
   const uint64_t *root_bits = q->bits + static_cast<uint64_t>(lv[0].u) * words;
@@ -290,20 +439,34 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   compact_bits_kernel<<<grid_for(words), 256, 0, s>>>(root_bits, words, ws.dscan[1], ws.rows[1]);
   kt.end(CSM_K_AUX, 16ull * words + 4ull * n1);
 
+  uint64_t *pin = ctx->pinned;
+  if (rows2) {
+    kt.begin(CSM_K_COUNT);
+    count_rows_kernel<<<grid_for(n1), kTile, 0, s>>>(probe, ws.rows[1], 1, n1, lv[L], cand_last, ws.cnt,
+                                                    reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCount));
+    kt.end(CSM_K_COUNT, n1 * 8ull);
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    kt.flush();
+    res.count = std::min<uint64_t>(pin[kCtlCount], cap);
+    res.capped = pin[kCtlCount] > cap;
+    finish_time();
+    return CSM_OK;
+  }
+
   uint64_t nrows[CSM_MAX_QV + 1] = {};
   uint64_t cursor[CSM_MAX_QV + 1] = {};
   uint64_t next_b[CSM_MAX_QV + 1] = {}, next_items[CSM_MAX_QV + 1] = {}, next_base[CSM_MAX_QV + 1] = {};
   nrows[1] = n1;
-  uint64_t *pin = ctx->pinned;
 
-  // prefix of parent degrees for level k and the first chunk choice
+  // driver-degree prefix for level k and the first chunk choice
   auto prepare_level = [&](uint32_t k) -> csm_status {
-    const uint32_t par = lv[k].parent;
     kt.begin(CSM_K_AUX);
-    parent_degree_kernel<<<grid_for(nrows[k] + 1), 256, 0, s>>>(g->off, ws.rows[k], k, par, nrows[k], ws.dscan[k]);
+    anchor_degree_kernel<<<grid_for(nrows[k] + 1), 256, 0, s>>>(g->off, ws.rows[k], k, lv[k], nrows[k],
+                                                               ws.dscan[k], ws.drv[k]);
     scan_u64_exclusive(ws.dscan[k], nrows[k] + 1, ws.partials, nullptr, s);
     choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], 0, item_cap, ws.ctl + kCtlChunkA);
-    kt.end(CSM_K_AUX, nrows[k] * (4ull + 8 + 16));
+    kt.end(CSM_K_AUX, nrows[k] * (4ull * lv[k].nanchor + 8 * lv[k].nanchor + 17));
     CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     cursor[k] = 0;
@@ -312,15 +475,24 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     next_base[k] = pin[kCtlChunkA + 2];
     return CSM_OK;
   };
+  auto next_chunk = [&](uint32_t k, uint64_t from, bool read_count) -> csm_status {
+    choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], from, item_cap, ws.ctl + kCtlChunkB);
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    next_b[k] = pin[kCtlChunkB];
+    next_items[k] = pin[kCtlChunkB + 1];
+    next_base[k] = pin[kCtlChunkB + 2];
+    (void)read_count;
+    return CSM_OK;
+  };
 
   csm_status st = prepare_level(1);
   if (st != CSM_OK) return st;
   std::vector<uint32_t> host_rows;
   uint64_t total = 0;
   uint32_t k = 1;
-  const uint32_t last = nq - 1;  // level whose extension completes an embedding
-  bool stop = false;
-  while (!stop) {
+  bool stop_all = false;
+  while (!stop_all) {
     if (cursor[k] >= nrows[k]) {
       if (k == 1) break;
       --k;
@@ -328,10 +500,11 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     }
     const uint64_t a = cursor[k], b = next_b[k], items = next_items[k];
     ExpandArgs ea{};
-    ea.off = g->off;
-    ea.adj = g->adj;
+    ea.probe = probe;
     ea.rows = ws.rows[k];
     ea.dscan = ws.dscan[k];
+    ea.drv = ws.drv[k];
+    ea.tile_row = ws.tile_row;
     ea.cand = q->bits + static_cast<uint64_t>(lv[k].u) * words;
     ea.row_a = a;
     ea.row_b = b;
@@ -345,28 +518,38 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     ea.items = items;
     const uint32_t ntiles = ceil_div(items, kTile);
     const uint64_t rows_in = b - a;
+    const size_t lds_rows = 4ull * kTileRows * k;
+    kt.begin(CSM_K_AUX);
+    tile_rows_kernel<<<grid_for(rows_in), 256, 0, s>>>(ws.dscan[k], a, b, ntiles, ws.tile_row);
+    kt.end(CSM_K_AUX, rows_in * 16ull + 4ull * ntiles);
     res.items += items;
-    // algorithmic bytes of one extension step: rows read, parent offsets,
-    // (adjacency id + bitmap word) per item, rows written
-    const uint64_t step_in = rows_in * (4ull * k + 8) + items * 12ull;
-    if (k == last && !want_list) {
-      kt.begin(CSM_K_COUNT);
-      expand_kernel<kCount><<<ntiles, kTile, 0, s>>>(ea);
-      kt.end(CSM_K_COUNT, step_in);
+    // algorithmic bytes of one step: rows read, driver offsets, (adjacency id
+    // + candidate bitmap word) per item; the rows written are added below
+    const uint64_t step_in = rows_in * (4ull * k + 9) + items * 12ull;
+    if (k == stop && !want_list) {
+      if (fused) {
+        ea.lv_last = lv[L];
+        ea.cand_last = cand_last;
+        ea.cnt = ws.cnt;
+        kt.begin(CSM_K_COUNT);
+        expand_kernel<kFused><<<ntiles, kTile, lds_rows, s>>>(ea);
+        kt.end(CSM_K_COUNT, step_in);
+      } else {
+        kt.begin(CSM_K_COUNT);
+        expand_kernel<kCount><<<ntiles, kTile, lds_rows, s>>>(ea);
+        kt.end(CSM_K_COUNT, step_in);
+      }
+      reduce_add_kernel<<<1, 1024, 0, s>>>(ws.tiles, ntiles, ea.count);
       cursor[k] = b;
-      choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], b, item_cap, ws.ctl + kCtlChunkB);
-      CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
-      CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-      next_b[k] = pin[kCtlChunkB];
-      next_items[k] = pin[kCtlChunkB + 1];
-      next_base[k] = pin[kCtlChunkB + 2];
+      st = next_chunk(k, b, true);
+      if (st != CSM_OK) return st;
       total = pin[kCtlCount];
-      if (total > cap) stop = true;
+      if (total > cap) stop_all = true;
       continue;
     }
     // mark, scan, materialise into level k + 1
     kt.begin(CSM_K_EXPAND);
-    expand_kernel<kMark><<<ntiles, kTile, 0, s>>>(ea);
+    expand_kernel<kMark><<<ntiles, kTile, lds_rows, s>>>(ea);
     kt.end(CSM_K_EXPAND, step_in);
     kt.begin(CSM_K_AUX);
     scan_u64_exclusive(ws.tiles, ntiles, ws.partials, ws.ctl + kCtlTotal, s);
@@ -377,12 +560,12 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     ea.out = ws.rows[k + 1];
     if (produced) {
       kt.begin(CSM_K_MATERIALIZE);
-      expand_kernel<kMaterialize><<<ntiles, kTile, 0, s>>>(ea);
-      kt.end(CSM_K_MATERIALIZE, rows_in * (4ull * k + 8) + items * (4ull + 1) + produced * 4ull * (k + 1));
+      expand_kernel<kMaterialize><<<ntiles, kTile, lds_rows, s>>>(ea);
+      kt.end(CSM_K_MATERIALIZE, rows_in * (4ull * k + 9) + items * (4ull + 1) + produced * 4ull * (k + 1));
     }
     res.partial_rows += produced;
     cursor[k] = b;
-    if (k == last) {  // list mode: rows are complete embeddings in lexicographic order
+    if (k == L) {  // list mode: rows are complete embeddings in lexicographic order
       const uint64_t lim = std::min(cap, o.list_capacity);
       const uint64_t want = total < lim ? std::min(produced, lim - total) : 0;
       if (want) {
@@ -392,22 +575,14 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
                                         hipMemcpyDeviceToHost, s));
       }
       total += produced;
-      choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], b, item_cap, ws.ctl + kCtlChunkB);
-      CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
-      CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-      next_b[k] = pin[kCtlChunkB];
-      next_items[k] = pin[kCtlChunkB + 1];
-      next_base[k] = pin[kCtlChunkB + 2];
-      if (total > cap) stop = true;
+      st = next_chunk(k, b, false);
+      if (st != CSM_OK) return st;
+      if (total > cap) stop_all = true;
       continue;
     }
     if (cursor[k] < nrows[k]) {
-      choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], b, item_cap, ws.ctl + kCtlChunkB);
-      CSM_HIP_TRY(ctx, hipMemcpyAsync(pin + kCtlChunkB, ws.ctl + kCtlChunkB, 24, hipMemcpyDeviceToHost, s));
-      CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-      next_b[k] = pin[kCtlChunkB];
-      next_items[k] = pin[kCtlChunkB + 1];
-      next_base[k] = pin[kCtlChunkB + 2];
+      st = next_chunk(k, b, false);
+      if (st != CSM_OK) return st;
     }
     if (produced) {
       nrows[k + 1] = produced;
@@ -435,7 +610,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
       std::copy(qrows.begin() + idx[i] * nq, qrows.begin() + (idx[i] + 1) * nq, o.list + i * nq);
     res.listed = nl;
   }
-  res.ms_join = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  finish_time();
   return CSM_OK;
 }
 

@@ -137,7 +137,64 @@ __global__ void max_degree_kernel(const uint32_t *off, uint32_t nv, uint32_t *ou
 
 uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(65536, (n + 255) / 256 + 1)); }
 
+// One workgroup per hub: set the bit of every neighbour.
+__global__ void hub_bits_kernel(const uint32_t *off, const uint32_t *adj, const uint32_t *hubs, uint32_t words,
+                                uint64_t *bits) {
+  const uint32_t h = blockIdx.x;
+  const uint32_t x = hubs[h];
+  uint64_t *row = bits + static_cast<uint64_t>(h) * words;
+  for (uint32_t o = off[x] + threadIdx.x; o < off[x + 1]; o += blockDim.x) {
+    const uint32_t w = adj[o];
+    atomicOr(reinterpret_cast<unsigned long long *>(&row[w >> 6]), 1ull << (w & 63));
+  }
+}
+
 }  // namespace
+
+// Hubs: vertices of degree >= kHubMinDegree, highest degree first, while the
+// bitmaps fit in the budget (1/16 of the arena, at most 4 GiB).
+csm_status build_hubs(csm_ctx *ctx, csm_graph *g) {
+  constexpr uint32_t kHubMinDegree = 128;
+  hipStream_t s = ctx->stream;
+  const uint32_t nv = g->n;
+  const uint32_t words = ceil_div(nv, 64);
+  std::vector<uint32_t> off(nv + 1ull);
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(off.data(), g->off, 4ull * (nv + 1ull), hipMemcpyDeviceToHost, s));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  std::vector<uint32_t> cand;
+  for (uint32_t v = 0; v < nv; ++v)
+    if (off[v + 1] - off[v] >= kHubMinDegree) cand.push_back(v);
+  std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+    uint32_t da = off[a + 1] - off[a], db = off[b + 1] - off[b];
+    return da != db ? da > db : a < b;
+  });
+  const uint64_t budget = std::min<uint64_t>(ctx->arena.capacity() / 16, 4ull << 30);
+  const uint64_t per = 8ull * words;
+  size_t nh = std::min<size_t>(cand.size(), per ? budget / per : 0);
+  cand.resize(nh);
+  g->hub_id = static_cast<int32_t *>(ctx->arena.alloc(4ull * std::max<uint32_t>(nv, 1)));
+  if (!g->hub_id) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (hub index)");
+  std::vector<int32_t> id(nv, -1);
+  for (size_t h = 0; h < nh; ++h) id[cand[h]] = static_cast<int32_t>(h);
+  if (nv) CSM_HIP_TRY(ctx, hipMemcpyAsync(g->hub_id, id.data(), 4ull * nv, hipMemcpyHostToDevice, s));
+  g->num_hubs = static_cast<uint32_t>(nh);
+  g->hub_min_degree = nh ? off[cand.back() + 1] - off[cand.back()] : 0;
+  if (nh) {
+    g->hub_bits = static_cast<uint64_t *>(ctx->arena.alloc(per * nh));
+    uint32_t *d_hubs = static_cast<uint32_t *>(ctx->arena.alloc(4ull * nh));
+    if (!g->hub_bits || !d_hubs) {
+      ctx->arena.free(d_hubs);
+      return set_error(ctx, CSM_E_NOMEM, "arena exhausted (hub bitmaps)");
+    }
+    CSM_HIP_TRY(ctx, hipMemsetAsync(g->hub_bits, 0, per * nh, s));
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(d_hubs, cand.data(), 4ull * nh, hipMemcpyHostToDevice, s));
+    hub_bits_kernel<<<static_cast<uint32_t>(nh), 256, 0, s>>>(g->off, g->adj, d_hubs, words, g->hub_bits);
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    ctx->arena.free(d_hubs);
+  }
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return CSM_OK;
+}
 
 csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels, uint64_t m,
                                const uint32_t *edges, csm_graph **out) {
@@ -159,6 +216,8 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
     ctx->arena.free(g->label);
     ctx->arena.free(g->off);
     ctx->arena.free(g->adj);
+    ctx->arena.free(g->hub_id);
+    ctx->arena.free(g->hub_bits);
     return set_error(ctx, st, msg);
   };
   if (nv) CSM_HIP_TRY(ctx, hipMemcpyAsync(g->label, labels, 4ull * nv, hipMemcpyHostToDevice, s));
@@ -169,6 +228,8 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
     fill_off_empty<<<grid_for(nv + 1ull), 256, 0, s>>>(g->off, nv);
     g->adj = static_cast<uint32_t *>(ctx->arena.alloc(4));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    csm_status hst = build_hubs(ctx, g.get());
+    if (hst != CSM_OK) return hst;
     *out = g.release();
     return CSM_OK;
   }
@@ -233,6 +294,8 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
   free_tmp();
   g->m = total / 2;
   g->max_degree = maxdeg;
+  csm_status hst = build_hubs(ctx, g.get());
+  if (hst != CSM_OK) return fail(hst, ctx->last_error);
   *out = g.release();
   return CSM_OK;
 }
@@ -288,6 +351,9 @@ csm_status query_build_desc(const csm_query_plan &plan, uint32_t graph_n, QueryD
     }
     for (uint32_t j = 0; j < p; ++j)
       if (plan.labels[plan.match_vertex[j]] == plan.labels[L.u]) L.inj[L.ninj++] = j;
+    L.nanchor = p == 0 ? 0 : 1 + L.nback;
+    L.anchor[0] = L.parent;
+    for (uint32_t t = 0; t < L.nback; ++t) L.anchor[1 + t] = L.back[t];
   }
   return CSM_OK;
 }
@@ -386,6 +452,8 @@ csm_status csm_graph_free(csm_ctx *ctx, csm_graph *g) {
   ctx->arena.free(g->off);
   ctx->arena.free(g->adj);
   ctx->arena.free(g->label);
+  ctx->arena.free(g->hub_id);
+  ctx->arena.free(g->hub_bits);
   delete g;
   return CSM_OK;
 }

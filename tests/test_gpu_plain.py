@@ -215,3 +215,44 @@ def test_config2_rmat16_counts(ctx):
         assert (res.count, bool(res.capped)) == (oc, ocap)
         q.free()
     g.free()
+
+
+def test_hub_probes_and_fused_counts(ctx):
+    """Graphs with hub vertices (dense-bitmap probes) and queries with and
+    without non-tree edges at the last position (fused count vs count kernel)."""
+    r = random.Random(8)
+    n = 700
+    edges = set()
+    for h in range(6):
+        for v in r.sample(range(6, n), 250 + 40 * h):
+            edges.add((h, v))
+    for _ in range(1500):
+        a, b = r.sample(range(n), 2)
+        edges.add((min(a, b), max(a, b)))
+    de = list(edges)
+    dl = [r.randrange(2) for _ in range(n)]
+    d = O.Data.from_arrays(dl, de)
+    g = ctx.upload_edges(dl, de)
+    assert g.info()[2] >= 128
+    seen_fused = seen_back = 0
+    for it in range(30):
+        nq = r.randint(2, 5)
+        ql, qe = rand_query(r, nq, r.randint(0, 3), 2)
+        oq = O.Query.from_arrays(ql, qe)
+        op = oq.plan()
+        oc, ocap, _, _, _ = O.match(d, oq, op, cap=50_000_000, threads=8)
+        plan = csm.HostGraph.from_edges(ql, qe).plan()
+        last = plan.num_v - 1
+        if plan.num_back[last]:
+            seen_back += 1
+        else:
+            seen_fused += 1
+        q = g.prepare(plan)
+        res, _ = q.match(cap=50_000_000)
+        print("hub case", it, nq, len(qe), oc, flush=True)
+        assert (res.count, bool(res.capped)) == (oc, ocap), it
+        res, _ = q.match(cap=50_000_000, chunk_items=4096)
+        assert (res.count, bool(res.capped)) == (oc, ocap), it
+        q.free()
+    assert seen_fused and seen_back
+    g.free()
