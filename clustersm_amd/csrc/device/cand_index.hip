@@ -1,0 +1,181 @@
+// Build of the candidate adjacency index (see cand_index.hpp).
+//
+// Two passes over the candidates of every slot (query edge s -> t): count
+// |N(x) ∩ C(t)| into the slot's offset array at x's rank, one exclusive scan
+// over all slots at once (offsets become absolute positions in cn_adj), then
+// write the lists.  A wave handles one 64-bit word of C(s); vertices above
+// kLightDegree are scanned by the whole wave, compacting in order with a
+// ballot, so every list stays sorted.
+#include <algorithm>
+
+#include "cand_index.hpp"
+
+namespace csm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr uint32_t kLightDegree = 64;
+
+struct SlotDesc {
+  uint32_t src, dst;
+  uint64_t base;    // first entry of the slot in cn_off
+  uint64_t ncand;   // |C(src)|
+};
+
+__global__ void rank_words_kernel(const uint64_t *bits, uint32_t words, uint32_t nq, uint64_t *out) {
+  const uint64_t total = static_cast<uint64_t>(nq) * (words + 1);
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += 256ull * gridDim.x) {
+    const uint64_t u = i / (words + 1), b = i % (words + 1);
+    out[i] = b < words ? static_cast<uint64_t>(__popcll(bits[u * words + b])) : 0;
+  }
+}
+
+__device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
+
+template <bool FILL>
+__global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+                                                      uint32_t words, const uint64_t *__restrict__ bits,
+                                                      const uint64_t *__restrict__ rank_pref,
+                                                      const SlotDesc *__restrict__ slots, uint64_t *__restrict__ cn_off,
+                                                      uint32_t *__restrict__ cn_adj) {
+  const SlotDesc sd = slots[blockIdx.y];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint64_t *xrow = bits + static_cast<uint64_t>(sd.src) * words;
+  const uint64_t *yrow = bits + static_cast<uint64_t>(sd.dst) * words;
+  const uint64_t *pref = rank_pref + static_cast<uint64_t>(sd.src) * (words + 1);
+  const uint64_t pref0 = pref[0];
+  if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cn_off[sd.base + sd.ncand] = 0;
+  for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {
+    const uint64_t xw = xrow[b];
+    if (xw == 0) continue;
+    const bool act = (xw >> lane) & 1ull;
+    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
+    const uint64_t slot = sd.base + (pref[b] - pref0) + lanes_below(xw);
+    uint32_t o0 = 0, dv = 0;
+    if (act) {
+      o0 = off[v];
+      dv = off[v + 1] - o0;
+    }
+    const bool light = act && dv <= kLightDegree;
+    if (!FILL) {
+      uint32_t c = 0;
+      if (light)
+        for (uint32_t o = o0; o < o0 + dv; ++o) c += bit_of(yrow, adj[o]);
+      uint64_t heavy = ballot(act && !light);
+      while (heavy) {
+        const int h = __builtin_ctzll(heavy);
+        heavy &= heavy - 1;
+        const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
+        uint32_t part = 0;
+        for (uint32_t o = ho + lane; o < ho + hd; o += kWave) part += bit_of(yrow, adj[o]);
+        for (int s = kWave / 2; s > 0; s >>= 1) part += __shfl_xor(part, s, kWave);
+        if (lane == h) c = part;
+      }
+      if (act) cn_off[slot] = c;
+    } else {
+      const uint64_t start = act ? cn_off[slot] : 0;
+      if (light) {
+        uint64_t p = start;
+        for (uint32_t o = o0; o < o0 + dv; ++o) {
+          const uint32_t w = adj[o];
+          if (bit_of(yrow, w)) cn_adj[p++] = w;
+        }
+      }
+      uint64_t heavy = ballot(act && !light);
+      while (heavy) {
+        const int h = __builtin_ctzll(heavy);
+        heavy &= heavy - 1;
+        const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
+        uint64_t p = __shfl(start, h, kWave);
+        for (uint32_t o = ho; o < ho + hd; o += kWave) {
+          bool f = false;
+          uint32_t w = 0;
+          if (o + lane < ho + hd) {
+            w = adj[o + lane];
+            f = bit_of(yrow, w);
+          }
+          const uint64_t m = ballot(f);
+          if (f) cn_adj[p + lanes_below(m)] = w;
+          p += static_cast<uint64_t>(__popcll(m));
+        }
+      }
+    }
+  }
+}
+
+uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(65536, (n + 255) / 256 + 1)); }
+
+}  // namespace
+
+csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
+                            std::vector<void *> &blocks, KernelTimer &kt) {
+  hipStream_t s = ctx->stream;
+  const uint32_t nq = q->desc.nq, words = q->desc.words;
+  auto get = [&](uint64_t bytes) {
+    void *p = ctx->arena.alloc(bytes);
+    if (p) blocks.push_back(p);
+    return p;
+  };
+  // slots: one per (position k >= 1, anchor a)
+  std::vector<SlotDesc> slots;
+  uint64_t entries = 0;
+  for (uint32_t k = 1; k < nq; ++k) {
+    const LevelDesc &L = q->level[k];
+    LevelJoin &J = ci.level[k];
+    std::memset(&J, 0, sizeof(J));
+    J.u = L.u;
+    J.parent = L.parent;
+    J.nanchor = L.nanchor;
+    J.ninj = L.ninj;
+    for (uint32_t t = 0; t < L.ninj; ++t) J.inj[t] = L.inj[t];
+    for (uint32_t a = 0; a < L.nanchor; ++a) {
+      const uint32_t src = q->level[L.anchor[a]].u;
+      J.anchor[a] = L.anchor[a];
+      J.src_u[a] = src;
+      J.off_base[a] = entries;
+      slots.push_back({src, L.u, entries, q->counts[src]});
+      entries += q->counts[src] + 1;
+    }
+  }
+  std::memset(&ci.level[0], 0, sizeof(LevelJoin));
+  ci.level[0].u = q->level[0].u;
+  const uint64_t rank_n = static_cast<uint64_t>(nq) * (words + 1);
+  const uint64_t scan_n = std::max<uint64_t>(rank_n, entries);
+  ci.rank_pref = static_cast<uint64_t *>(get(8 * rank_n));
+  ci.cn_off = static_cast<uint64_t *>(get(8 * std::max<uint64_t>(entries, 1)));
+  uint64_t *partials = static_cast<uint64_t *>(get(8 * (scan_n / kScanTile + 2)));
+  uint64_t *total_d = static_cast<uint64_t *>(get(64));
+  SlotDesc *d_slots = static_cast<SlotDesc *>(get(sizeof(SlotDesc) * std::max<size_t>(slots.size(), 1)));
+  if (!ci.rank_pref || !ci.cn_off || !partials || !total_d || !d_slots)
+    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate index)");
+  kt.begin(CSM_K_AUX);
+  rank_words_kernel<<<grid_for(rank_n), 256, 0, s>>>(q->bits, words, nq, ci.rank_pref);
+  scan_u64_exclusive(ci.rank_pref, rank_n, partials, nullptr, s);
+  kt.end(CSM_K_AUX, 16ull * rank_n);
+  if (slots.empty()) return CSM_OK;
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(d_slots, slots.data(), sizeof(SlotDesc) * slots.size(), hipMemcpyHostToDevice, s));
+  const dim3 grid(std::max<uint32_t>(1, std::min<uint32_t>(1024, ceil_div(words, kWaves))),
+                  static_cast<uint32_t>(slots.size()));
+  uint64_t cand_words = 0;
+  for (auto &sd : slots) cand_words += sd.ncand;
+  kt.begin(CSM_K_AUX);
+  cn_kernel<false><<<grid, kThreads, 0, s>>>(g->off, g->adj, words, q->bits, ci.rank_pref, d_slots, ci.cn_off,
+                                             nullptr);
+  scan_u64_exclusive(ci.cn_off, entries, partials, total_d, s);
+  kt.end(CSM_K_AUX, 16ull * words * slots.size() + 16ull * entries);
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, total_d, 8, hipMemcpyDeviceToHost, s));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  ci.total_lists = ctx->pinned[0];
+  ci.cn_adj = static_cast<uint32_t *>(get(4 * std::max<uint64_t>(ci.total_lists, 1)));
+  if (!ci.cn_adj) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate lists)");
+  kt.begin(CSM_K_AUX);
+  cn_kernel<true><<<grid, kThreads, 0, s>>>(g->off, g->adj, words, q->bits, ci.rank_pref, d_slots, ci.cn_off,
+                                            ci.cn_adj);
+  kt.end(CSM_K_AUX, 8ull * words * slots.size() + 8ull * cand_words + 4ull * ci.total_lists);
+  ci.build_bytes = 24ull * words * slots.size() + 8ull * cand_words + 16ull * entries + 4ull * ci.total_lists;
+  return CSM_OK;
+}
+
+}  // namespace csm

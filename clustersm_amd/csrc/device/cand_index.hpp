@@ -1,0 +1,59 @@
+// Candidate adjacency index of one query: for every query edge, oriented from
+// the earlier-matched endpoint s to the later one t, and every x in C(s), the
+// sorted list N(x) ∩ C(t).  Lists of a query are concatenated in one array;
+// the list of x is found through x's rank among the set bits of C(s).
+#pragma once
+
+#include "common.hpp"
+
+namespace csm {
+
+// Position k of the matching order as the join kernels read it.
+struct LevelJoin {
+  uint32_t u;                         // query vertex matched at this position
+  uint32_t parent;                    // parent position
+  uint32_t nanchor;                   // parent + non-tree neighbours
+  uint32_t ninj;                      // earlier positions with the same label
+  uint32_t anchor[CSM_MAX_QV + 1];    // anchor positions (anchor[0] = parent)
+  uint32_t inj[CSM_MAX_QV];
+  uint32_t src_u[CSM_MAX_QV + 1];     // query vertex at each anchor
+  uint64_t off_base[CSM_MAX_QV + 1];  // start of the anchor's slot in cn_off
+};
+
+struct CandIndex {
+  const uint64_t *bits;       // nq x words candidate bitmaps
+  const uint64_t *rank_pref;  // nq x (words + 1): exclusive popcount prefix, global over all rows
+  const uint64_t *cn_off;     // per slot |C(s)| + 1 absolute list offsets into cn_adj
+  const uint32_t *cn_adj;
+  uint32_t words;
+
+  // rank of x among the set bits of C(u); x must be a candidate of u
+  __device__ inline uint64_t rank(uint32_t u, uint32_t x) const {
+    const uint64_t *pref = rank_pref + static_cast<uint64_t>(u) * (words + 1);
+    const uint32_t b = x >> 6;
+    const uint64_t w = bits[static_cast<uint64_t>(u) * words + b];
+    return pref[b] - pref[0] + static_cast<uint64_t>(__popcll(w & ((1ull << (x & 63)) - 1)));
+  }
+  // [begin, end) of N(x) ∩ C(u_k) for anchor a of position k
+  __device__ inline void list(const LevelJoin &L, uint32_t a, uint32_t x, uint64_t &beg, uint64_t &end) const {
+    const uint64_t *o = cn_off + L.off_base[a] + rank(L.src_u[a], x);
+    beg = o[0];
+    end = o[1];
+  }
+};
+
+struct CandIndexHost {
+  uint64_t *rank_pref = nullptr;
+  uint64_t *cn_off = nullptr;
+  uint32_t *cn_adj = nullptr;
+  uint64_t total_lists = 0;   // entries of cn_adj
+  uint64_t build_bytes = 0;   // algorithmic bytes of the build
+  LevelJoin level[CSM_MAX_QV];
+};
+
+// Builds the index of q (candidates must be filtered) with arena memory owned
+// by `blocks` (freed by the caller).  Launches on ctx->stream and synchronises.
+csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
+                            std::vector<void *> &blocks, KernelTimer &kt);
+
+}  // namespace csm

@@ -5,22 +5,23 @@
 // chunks".  Level k holds rows of k vertex ids (matching-order positions
 // 0..k-1, row-major).
 //
-// Extending a row to position k needs a vertex adjacent to the row's vertex
-// at every anchor of k (its tree parent and its non-tree neighbours).  Each
-// row enumerates the shortest of those adjacency lists (the "driver") and
-// probes the others (hub bitmap or binary search).  Every (row, driver
-// neighbour) pair is one item; items are numbered through the exclusive
-// prefix sum of the drivers' degrees, so a workgroup tile of 256 consecutive
-// items covers a contiguous run of rows.
+// Extending a row to position k needs a candidate of u_k adjacent to the
+// row's vertex at every anchor of k (its tree parent and its non-tree
+// neighbours).  Each row enumerates the shortest of its anchors' candidate
+// lists N(x) ∩ C(u_k) (the "driver", from the candidate adjacency index) and
+// probes the other anchors' adjacency (hub bitmap or binary search).  Every
+// (row, driver entry) pair is one item; items are numbered through the
+// exclusive prefix sum of the driver list lengths, so a workgroup tile of 256
+// consecutive items covers a contiguous run of rows.
 //
 //   expand<MARK>        test each item, 64-bit survivor mask per wave
 //   scan                tile survivor counts -> output offsets
 //   expand<MATERIALIZE> write surviving rows (prefix + new vertex), item order
 //   expand<COUNT>       last position: count survivors
 //   expand<FUSED>       last position without non-tree edges, counted from the
-//                       position before it: each surviving item adds
-//                       |N(x) ∩ C(u_last)| (precomputed per vertex x) minus
-//                       the row's own vertices in that set
+//                       position before it: each surviving item adds the
+//                       length of N(x) ∩ C(u_last) for its parent vertex x
+//                       minus the row's own vertices in that list
 //
 // Item order is (row, ascending new vertex): every level is in lexicographic
 // order of the matching-order tuple and DFS over chunks enumerates in that
@@ -29,7 +30,7 @@
 #include <chrono>
 #include <numeric>
 
-#include "common.hpp"
+#include "cand_index.hpp"
 
 namespace csm {
 
@@ -39,41 +40,29 @@ namespace {
 
 constexpr int kTile = 256;          // items per workgroup
 constexpr int kTileWaves = kTile / kWave;
-constexpr uint32_t kLightDegree = 64;
 
 struct ExpandArgs {
   AdjProbe probe;
+  CandIndex ci;
   const uint32_t *rows;     // level k, stride k
-  const uint64_t *dscan;    // level k driver-degree prefix, n_k + 1 entries
+  const uint64_t *dscan;    // level k driver-length prefix, n_k + 1 entries
   const uint8_t *drv;       // driver anchor index per row
   const uint32_t *tile_row; // first row of each tile, relative to row_a (ntiles + 1 entries)
-  const uint64_t *cand;     // candidate bitmap row of the new vertex
   uint64_t row_a, row_b;    // chunk rows [a, b)
   uint64_t item_base;       // dscan[a]
   uint64_t items;           // dscan[b] - dscan[a]
   uint32_t k;               // columns of the input rows = position being matched
-  LevelDesc lv;             // position k
+  LevelJoin lv;             // position k
   uint64_t *mask;           // MARK: items / 64 words
-  uint64_t *tile_cnt;       // MARK: survivors per tile
+  uint64_t *tile_cnt;       // MARK / COUNT / FUSED: per tile
   const uint64_t *tile_off; // MATERIALIZE: exclusive prefix of tile_cnt
   uint32_t *out;            // MATERIALIZE: rows of k + 1 ids
-  unsigned long long *count;  // COUNT / FUSED
-  // FUSED: the last position (k + 1)
-  LevelDesc lv_last;
+  unsigned long long *count;  // COUNT / FUSED total
+  LevelJoin lv_last;        // FUSED: the last position (k + 1)
   const uint64_t *cand_last;
-  const uint32_t *cnt;      // |N(x) ∩ C(u_last)| per vertex x
 };
 
 enum { kMark = 0, kCount = 1, kMaterialize = 2, kFused = 3 };
-
-// Largest r in [lo, hi) with dscan[r] <= x (dscan[lo] <= x < dscan[hi]).
-__device__ inline uint64_t upper_row(const uint64_t *dscan, uint64_t lo, uint64_t hi, uint64_t x) {
-  while (hi - lo > 1) {
-    uint64_t mid = lo + (hi - lo) / 2;
-    if (dscan[mid] <= x) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 __device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
 
@@ -88,29 +77,45 @@ __device__ inline T block_sum(T v, T *lds) {
   return t;
 }
 
-constexpr int kTileRows = kTile + 2;  // rows a tile can touch (each row has >= 1 item)
+constexpr int kTileRows = kTile + 2;  // rows staged in LDS per tile
 
-// Dynamic LDS: the tile's rows (kTileRows x k ids).
+// Largest r in [lo, hi) with dscan[r] <= x (dscan[lo] <= x < dscan[hi]).
+__device__ inline uint64_t upper_row(const uint64_t *dscan, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (dscan[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Dynamic LDS: the tile's rows (kTileRows x k ids).  A tile whose items span
+// more than kTileRows rows (rows with empty driver lists in between) reads
+// its rows from global memory instead.
 template <int MODE>
 __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   extern __shared__ uint32_t s_rows[];
   __shared__ uint64_t sd[kTileRows + 1];
-  __shared__ uint32_t s_base[kTileRows];
+  __shared__ uint64_t s_base[kTileRows];
   __shared__ uint32_t s_wcnt[kTileWaves];
   __shared__ unsigned long long s_sum[kTileWaves];
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kTile;
   const uint64_t rlo = a.row_a + a.tile_row[blockIdx.x];
   const uint64_t rend = min(a.row_a + a.tile_row[blockIdx.x + 1] + 1, a.row_b);  // exclusive
-  const uint32_t nr = static_cast<uint32_t>(rend - rlo);
+  const uint64_t span = rend - rlo;
+  const bool wide = span > static_cast<uint64_t>(kTileRows);
+  const uint32_t nr = wide ? 0u : static_cast<uint32_t>(span);
   const uint32_t k = a.k;
-  // stage: driver-degree prefix, rows (coalesced), driver list offset per row
-  for (uint32_t i = tid; i <= nr; i += kTile) sd[i] = a.dscan[rlo + i];
-  for (uint32_t i = tid; i < nr * k; i += kTile) s_rows[i] = a.rows[rlo * k + i];
-  __syncthreads();
-  for (uint32_t i = tid; i < nr; i += kTile) {
-    const uint32_t xd = s_rows[i * k + a.lv.anchor[a.drv[rlo + i]]];
-    s_base[i] = a.probe.off[xd];
+  if (!wide) {  // stage: driver-length prefix, rows (coalesced), driver list start per row
+    for (uint32_t i = tid; i <= nr; i += kTile) sd[i] = a.dscan[rlo + i];
+    for (uint32_t i = tid; i < nr * k; i += kTile) s_rows[i] = a.rows[rlo * k + i];
+    __syncthreads();
+    for (uint32_t i = tid; i < nr; i += kTile) {
+      const uint32_t d = a.drv[rlo + i];
+      uint64_t beg, end;
+      a.ci.list(a.lv, d, s_rows[i * k + a.lv.anchor[d]], beg, end);
+      s_base[i] = beg;
+    }
   }
   __syncthreads();
 
@@ -118,30 +123,41 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   const bool valid = item < a.items;
   bool ok = false;
   uint32_t w = 0;
-  uint32_t ri = 0;
+  const uint32_t *row = nullptr;
   unsigned long long contrib = 0;
   if (valid) {
     const uint64_t gi = a.item_base + item;
-    uint32_t lo = 0, hi = nr;  // sd[lo] <= gi < sd[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (sd[mid] <= gi) lo = mid; else hi = mid;
+    uint32_t d;
+    if (!wide) {
+      uint32_t lo = 0, hi = nr;  // sd[lo] <= gi < sd[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sd[mid] <= gi) lo = mid; else hi = mid;
+      }
+      row = s_rows + lo * k;
+      d = a.drv[rlo + lo];
+      w = a.ci.cn_adj[s_base[lo] + (gi - sd[lo])];
+    } else {
+      const uint64_t r = upper_row(a.dscan, rlo, rend, gi);
+      row = a.rows + r * k;
+      d = a.drv[r];
+      uint64_t beg, end;
+      a.ci.list(a.lv, d, row[a.lv.anchor[d]], beg, end);
+      w = a.ci.cn_adj[beg + (gi - a.dscan[r])];
     }
-    ri = lo;
-    const uint32_t *row = s_rows + ri * k;
-    w = a.probe.adj[s_base[ri] + static_cast<uint32_t>(gi - sd[ri])];
     if (MODE == kMaterialize) {
       ok = (a.mask[item >> 6] >> (item & 63)) & 1ull;
     } else {
-      const uint32_t d = a.drv[rlo + ri];
-      ok = bit_of(a.cand, w);
+      ok = true;  // w is a candidate of u_k adjacent to the driver anchor
       for (uint32_t t = 0; ok && t < a.lv.ninj; ++t) ok = row[a.lv.inj[t]] != w;
       for (uint32_t t = 0; ok && t < a.lv.nanchor; ++t)
         if (t != d) ok = a.probe.has(row[a.lv.anchor[t]], w);
       if (MODE == kFused && ok) {
         const uint32_t pk = a.lv_last.parent;
         const uint32_t x = pk == k ? w : row[pk];
-        long long c = a.cnt[x];
+        uint64_t lb, le;
+        a.ci.list(a.lv_last, 0, x, lb, le);
+        long long c = static_cast<long long>(le - lb);
         for (uint32_t t = 0; t < a.lv_last.ninj; ++t) {
           const uint32_t j = a.lv_last.inj[t];
           const uint32_t v = j == k ? w : row[j];
@@ -163,7 +179,6 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
     if (ok) {
       uint64_t pos = a.tile_off[blockIdx.x] + lanes_below(m);
       for (int x = 0; x < wave; ++x) pos += s_wcnt[x];
-      const uint32_t *row = s_rows + ri * k;
       uint32_t *dst = a.out + pos * (k + 1);
       for (uint32_t j = 0; j < k; ++j) dst[j] = row[j];
       dst[k] = w;
@@ -208,81 +223,29 @@ __global__ void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, 
   if (blockIdx.x == 0 && threadIdx.x == 0) tile_row[ntiles] = static_cast<uint32_t>(b - a);
 }
 
-// dscan[i] = smallest degree among the row's anchor vertices, drv[i] = that
+// dscan[i] = shortest candidate list among the row's anchors, drv[i] = that
 // anchor's index; dscan[n] = 0.
-__global__ void anchor_degree_kernel(const uint32_t *off, const uint32_t *rows, uint32_t k, LevelDesc lv, uint64_t n,
-                                     uint64_t *dscan, uint8_t *drv) {
+__global__ void anchor_len_kernel(CandIndex ci, const uint32_t *rows, uint32_t k, LevelJoin lv, uint64_t n,
+                                  uint64_t *dscan, uint8_t *drv) {
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i <= n; i += 256ull * gridDim.x) {
     if (i == n) {
       dscan[i] = 0;
       continue;
     }
     const uint32_t *row = rows + i * k;
-    uint32_t best = 0xFFFFFFFFu, bi = 0;
+    uint64_t best = ~0ull;
+    uint32_t bi = 0;
     for (uint32_t t = 0; t < lv.nanchor; ++t) {
-      const uint32_t x = row[lv.anchor[t]];
-      const uint32_t dg = off[x + 1] - off[x];
-      if (dg < best) {
-        best = dg;
+      uint64_t beg, end;
+      ci.list(lv, t, row[lv.anchor[t]], beg, end);
+      if (end - beg < best) {
+        best = end - beg;
         bi = t;
       }
     }
     dscan[i] = best;
     drv[i] = static_cast<uint8_t>(bi);
   }
-}
-
-// cnt[v] = |N(v) ∩ C(u_last)| for v in C(u_parent), 0 elsewhere.
-__global__ __launch_bounds__(kTile) void cand_nbr_count_kernel(const uint32_t *__restrict__ off,
-                                                               const uint32_t *__restrict__ adj, uint32_t n,
-                                                               uint32_t words, const uint64_t *__restrict__ xbits,
-                                                               const uint64_t *__restrict__ ybits,
-                                                               uint32_t *__restrict__ cnt) {
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (uint32_t b = blockIdx.x * kTileWaves + wave; b < words; b += gridDim.x * kTileWaves) {
-    const uint64_t xw = xbits[b];
-    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
-    const bool act = (xw >> lane) & 1ull;
-    uint32_t o0 = 0, dv = 0, c = 0;
-    if (act) {
-      o0 = off[v];
-      dv = off[v + 1] - o0;
-    }
-    if (act && dv <= kLightDegree)
-      for (uint32_t o = o0; o < o0 + dv; ++o) c += bit_of(ybits, adj[o]);
-    uint64_t heavy = ballot(act && dv > kLightDegree);
-    while (heavy) {
-      const int h = __builtin_ctzll(heavy);
-      heavy &= heavy - 1;
-      const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
-      uint32_t part = 0;
-      for (uint32_t o = ho + lane; o < ho + hd; o += kWave) part += bit_of(ybits, adj[o]);
-      for (int s = kWave / 2; s > 0; s >>= 1) part += __shfl_xor(part, s, kWave);
-      if (lane == h) c = part;
-    }
-    if (v < n) cnt[v] = c;
-  }
-}
-
-// Rows of the last level counted directly (queries of two vertices).
-__global__ __launch_bounds__(kTile) void count_rows_kernel(AdjProbe probe, const uint32_t *rows, uint32_t k,
-                                                           uint64_t n, LevelDesc lv_last,
-                                                           const uint64_t *cand_last, const uint32_t *cnt,
-                                                           unsigned long long *count) {
-  __shared__ unsigned long long s_sum[kTileWaves];
-  unsigned long long acc = 0;
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
-    const uint32_t *row = rows + i * k;
-    const uint32_t x = row[lv_last.parent];
-    long long c = cnt[x];
-    for (uint32_t t = 0; t < lv_last.ninj; ++t) {
-      const uint32_t v = row[lv_last.inj[t]];
-      if (v != x && bit_of(cand_last, v) && probe.has(x, v)) --c;
-    }
-    acc += static_cast<unsigned long long>(c);
-  }
-  const unsigned long long tot = block_sum(acc, s_sum);
-  if (threadIdx.x == 0 && tot) atomicAdd(count, tot);
 }
 
 __global__ void word_popcount_kernel(const uint64_t *bits, uint32_t words, uint64_t *cnt) {
@@ -339,7 +302,7 @@ struct Workspace {
   uint64_t *dscan[CSM_MAX_QV + 1] = {};
   uint8_t *drv[CSM_MAX_QV + 1] = {};
   uint64_t *mask = nullptr, *tiles = nullptr, *partials = nullptr, *ctl = nullptr;
-  uint32_t *cnt = nullptr, *tile_row = nullptr;
+  uint32_t *tile_row = nullptr;
   ~Workspace() {
     for (void *p : blocks) ctx->arena.free(p);
   }
@@ -387,7 +350,13 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   uint64_t per_item = 16;  // mask bit, tile counter, scan partials, rounded up
   for (uint32_t k = 2; k <= stop; ++k) per_item += 4ull * k + 8 + 1;
   if (want_list) per_item += 4ull * nq;
-  const uint64_t fixed = 13ull * (n1 + 1) + 8ull * (words + 1) + (precount ? 4ull * g->n : 0) + (1 << 20);
+  // candidate adjacency index first: its size is data dependent
+  CandIndexHost cih;
+  csm_status st = build_cand_index(ctx, g, q, cih, ws.blocks, kt);
+  if (st != CSM_OK) return st;
+  CandIndex ci{q->bits, cih.rank_pref, cih.cn_off, cih.cn_adj, words};
+  const LevelJoin *lj = cih.level;
+  const uint64_t fixed = 13ull * (n1 + 1) + 8ull * (words + 1) + (1 << 20);
   const uint64_t avail = ctx->arena.largest_free();
   if (avail < fixed + per_item * g->max_degree * 2)
     return set_error(ctx, CSM_E_NOMEM, "arena too small for the join workspace");
@@ -416,19 +385,21 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     ws.rows[nq] = static_cast<uint32_t *>(ws.get(4ull * nq * item_cap));
     okalloc = ws.rows[nq] != nullptr;
   }
-  if (okalloc && precount) {
-    ws.cnt = static_cast<uint32_t *>(ws.get(4ull * g->n));
-    okalloc = ws.cnt != nullptr;
-  }
   if (!okalloc) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (join workspace)");
   CSM_HIP_TRY(ctx, hipMemsetAsync(ws.ctl, 0, 8 * kCtlSize, s));
   const uint64_t *cand_last = q->bits + static_cast<uint64_t>(lv[L].u) * words;
-  if (precount) {
-    const uint64_t *xbits = q->bits + static_cast<uint64_t>(lv[lv[L].parent].u) * words;
-    kt.begin(CSM_K_AUX);
-    cand_nbr_count_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(words, kTileWaves))), kTile, 0,
-                            s>>>(g->off, g->adj, g->n, words, xbits, cand_last, ws.cnt);
-    kt.end(CSM_K_AUX, 8ull * words + 4ull * g->n + 12ull * q->counts[lv[lv[L].parent].u]);
+  uint64_t *pin = ctx->pinned;
+  if (rows2) {  // two vertices: the count is the total length of the one slot's lists
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, cih.cn_off + lj[1].off_base[0], 8, hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin + 1, cih.cn_off + lj[1].off_base[0] + n1, 8, hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    kt.flush();
+    const uint64_t total2 = pin[1] - pin[0];
+    res.count = std::min<uint64_t>(total2, cap);
+    res.capped = total2 > cap;
+    res.items = n1;
+    finish_time();
+    return CSM_OK;
   }
 
   // level 1: root candidates, ascending
@@ -439,20 +410,6 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   compact_bits_kernel<<<grid_for(words), 256, 0, s>>>(root_bits, words, ws.dscan[1], ws.rows[1]);
   kt.end(CSM_K_AUX, 16ull * words + 4ull * n1);
 
-  uint64_t *pin = ctx->pinned;
-  if (rows2) {
-    kt.begin(CSM_K_COUNT);
-    count_rows_kernel<<<grid_for(n1), kTile, 0, s>>>(probe, ws.rows[1], 1, n1, lv[L], cand_last, ws.cnt,
-                                                    reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCount));
-    kt.end(CSM_K_COUNT, n1 * 8ull);
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
-    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-    kt.flush();
-    res.count = std::min<uint64_t>(pin[kCtlCount], cap);
-    res.capped = pin[kCtlCount] > cap;
-    finish_time();
-    return CSM_OK;
-  }
 
   uint64_t nrows[CSM_MAX_QV + 1] = {};
   uint64_t cursor[CSM_MAX_QV + 1] = {};
@@ -462,8 +419,8 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   // driver-degree prefix for level k and the first chunk choice
   auto prepare_level = [&](uint32_t k) -> csm_status {
     kt.begin(CSM_K_AUX);
-    anchor_degree_kernel<<<grid_for(nrows[k] + 1), 256, 0, s>>>(g->off, ws.rows[k], k, lv[k], nrows[k],
-                                                               ws.dscan[k], ws.drv[k]);
+    anchor_len_kernel<<<grid_for(nrows[k] + 1), 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k],
+                                                            ws.drv[k]);
     scan_u64_exclusive(ws.dscan[k], nrows[k] + 1, ws.partials, nullptr, s);
     choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], 0, item_cap, ws.ctl + kCtlChunkA);
     kt.end(CSM_K_AUX, nrows[k] * (4ull * lv[k].nanchor + 8 * lv[k].nanchor + 17));
@@ -486,7 +443,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     return CSM_OK;
   };
 
-  csm_status st = prepare_level(1);
+  st = prepare_level(1);
   if (st != CSM_OK) return st;
   std::vector<uint32_t> host_rows;
   uint64_t total = 0;
@@ -501,21 +458,29 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     const uint64_t a = cursor[k], b = next_b[k], items = next_items[k];
     ExpandArgs ea{};
     ea.probe = probe;
+    ea.ci = ci;
     ea.rows = ws.rows[k];
     ea.dscan = ws.dscan[k];
     ea.drv = ws.drv[k];
     ea.tile_row = ws.tile_row;
-    ea.cand = q->bits + static_cast<uint64_t>(lv[k].u) * words;
     ea.row_a = a;
     ea.row_b = b;
     ea.k = k;
-    ea.lv = lv[k];
+    ea.lv = lj[k];
     ea.mask = ws.mask;
     ea.tile_cnt = ws.tiles;
     ea.tile_off = ws.tiles;
     ea.count = reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCount);
     ea.item_base = next_base[k];
     ea.items = items;
+    if (items == 0) {  // every row of the chunk has an empty driver list
+      cursor[k] = b;
+      if (cursor[k] < nrows[k]) {
+        st = next_chunk(k, b, false);
+        if (st != CSM_OK) return st;
+      }
+      continue;
+    }
     const uint32_t ntiles = ceil_div(items, kTile);
     const uint64_t rows_in = b - a;
     const size_t lds_rows = 4ull * kTileRows * k;
@@ -525,12 +490,11 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     res.items += items;
     // algorithmic bytes of one step: rows read, driver offsets, (adjacency id
     // + candidate bitmap word) per item; the rows written are added below
-    const uint64_t step_in = rows_in * (4ull * k + 9) + items * 12ull;
+    const uint64_t step_in = rows_in * (4ull * k + 9) + items * 4ull;
     if (k == stop && !want_list) {
       if (fused) {
-        ea.lv_last = lv[L];
+        ea.lv_last = lj[L];
         ea.cand_last = cand_last;
-        ea.cnt = ws.cnt;
         kt.begin(CSM_K_COUNT);
         expand_kernel<kFused><<<ntiles, kTile, lds_rows, s>>>(ea);
         kt.end(CSM_K_COUNT, step_in);
