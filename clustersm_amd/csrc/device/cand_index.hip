@@ -19,16 +19,28 @@ constexpr int kWaves = kThreads / kWave;
 constexpr uint32_t kLightDegree = 64;
 
 struct SlotDesc {
+  const uint64_t *bits;       // the slot's query
+  const uint64_t *rank_pref;
   uint32_t src, dst;
   uint64_t base;    // first entry of the slot in cn_off
   uint64_t ncand;   // |C(src)|
 };
 
-__global__ void rank_words_kernel(const uint64_t *bits, uint32_t words, uint32_t nq, uint64_t *out) {
-  const uint64_t total = static_cast<uint64_t>(nq) * (words + 1);
+struct RankJob {
+  const uint64_t *bits;
+  uint64_t *out;    // nq x (words + 1)
+  uint32_t nq;
+};
+
+// Per-word popcounts of every candidate bitmap (blockIdx.y = query); the
+// entry after each row's last word is 0, so one exclusive scan over the
+// concatenation gives every row's prefix.
+__global__ void rank_words_kernel(const RankJob *jobs, uint32_t words) {
+  const RankJob j = jobs[blockIdx.y];
+  const uint64_t total = static_cast<uint64_t>(j.nq) * (words + 1);
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += 256ull * gridDim.x) {
     const uint64_t u = i / (words + 1), b = i % (words + 1);
-    out[i] = b < words ? static_cast<uint64_t>(__popcll(bits[u * words + b])) : 0;
+    j.out[i] = b < words ? static_cast<uint64_t>(__popcll(j.bits[u * words + b])) : 0;
   }
 }
 
@@ -36,11 +48,11 @@ __device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v
 
 template <bool FILL>
 __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-                                                      uint32_t words, const uint64_t *__restrict__ bits,
-                                                      const uint64_t *__restrict__ rank_pref,
-                                                      const SlotDesc *__restrict__ slots, uint64_t *__restrict__ cn_off,
-                                                      uint32_t *__restrict__ cn_adj) {
+                                                      uint32_t words, const SlotDesc *__restrict__ slots,
+                                                      uint64_t *__restrict__ cn_off, uint32_t *__restrict__ cn_adj) {
   const SlotDesc sd = slots[blockIdx.y];
+  const uint64_t *bits = sd.bits;
+  const uint64_t *rank_pref = sd.rank_pref;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const uint64_t *xrow = bits + static_cast<uint64_t>(sd.src) * words;
   const uint64_t *yrow = bits + static_cast<uint64_t>(sd.dst) * words;
@@ -109,73 +121,108 @@ uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(
 
 }  // namespace
 
-csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
-                            std::vector<void *> &blocks, KernelTimer &kt) {
+csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
+                                  std::vector<CandIndexHost> &out, std::vector<void *> &blocks, KernelTimer &kt) {
   hipStream_t s = ctx->stream;
-  const uint32_t nq = q->desc.nq, words = q->desc.words;
+  const uint32_t nqs = static_cast<uint32_t>(qs.size());
+  if (nqs == 0) return CSM_OK;
+  const uint32_t words = qs[0]->desc.words;
+  out.assign(nqs, CandIndexHost());
   auto get = [&](uint64_t bytes) {
     void *p = ctx->arena.alloc(bytes);
     if (p) blocks.push_back(p);
     return p;
   };
-  // slots: one per (position k >= 1, anchor a)
+  // rank prefixes of every query in one buffer
+  std::vector<uint64_t> rank_base(nqs);
+  uint64_t rank_n = 0;
+  for (uint32_t i = 0; i < nqs; ++i) {
+    rank_base[i] = rank_n;
+    rank_n += static_cast<uint64_t>(qs[i]->desc.nq) * (words + 1);
+  }
+  uint64_t *rank_all = static_cast<uint64_t *>(get(8 * rank_n));
+  if (!rank_all) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate index ranks)");
+  // slots: one per (query, position k >= 1, anchor a)
   std::vector<SlotDesc> slots;
   uint64_t entries = 0;
-  for (uint32_t k = 1; k < nq; ++k) {
-    const LevelDesc &L = q->level[k];
-    LevelJoin &J = ci.level[k];
-    std::memset(&J, 0, sizeof(J));
-    J.u = L.u;
-    J.parent = L.parent;
-    J.nanchor = L.nanchor;
-    J.ninj = L.ninj;
-    for (uint32_t t = 0; t < L.ninj; ++t) J.inj[t] = L.inj[t];
-    for (uint32_t a = 0; a < L.nanchor; ++a) {
-      const uint32_t src = q->level[L.anchor[a]].u;
-      J.anchor[a] = L.anchor[a];
-      J.src_u[a] = src;
-      J.off_base[a] = entries;
-      slots.push_back({src, L.u, entries, q->counts[src]});
-      entries += q->counts[src] + 1;
+  for (uint32_t i = 0; i < nqs; ++i) {
+    const csm_query *q = qs[i];
+    CandIndexHost &ci = out[i];
+    ci.rank_pref = rank_all + rank_base[i];
+    std::memset(&ci.level[0], 0, sizeof(LevelJoin));
+    ci.level[0].u = q->level[0].u;
+    for (uint32_t k = 1; k < q->desc.nq; ++k) {
+      const LevelDesc &L = q->level[k];
+      LevelJoin &J = ci.level[k];
+      std::memset(&J, 0, sizeof(J));
+      J.u = L.u;
+      J.parent = L.parent;
+      J.nanchor = L.nanchor;
+      J.ninj = L.ninj;
+      for (uint32_t t = 0; t < L.ninj; ++t) J.inj[t] = L.inj[t];
+      for (uint32_t a = 0; a < L.nanchor; ++a) {
+        const uint32_t src = q->level[L.anchor[a]].u;
+        J.anchor[a] = L.anchor[a];
+        J.src_u[a] = src;
+        J.off_base[a] = entries;
+        slots.push_back({q->bits, ci.rank_pref, src, L.u, entries, q->counts[src]});
+        entries += q->counts[src] + 1;
+      }
     }
   }
-  std::memset(&ci.level[0], 0, sizeof(LevelJoin));
-  ci.level[0].u = q->level[0].u;
-  const uint64_t rank_n = static_cast<uint64_t>(nq) * (words + 1);
   const uint64_t scan_n = std::max<uint64_t>(rank_n, entries);
-  ci.rank_pref = static_cast<uint64_t *>(get(8 * rank_n));
-  ci.cn_off = static_cast<uint64_t *>(get(8 * std::max<uint64_t>(entries, 1)));
+  uint64_t *cn_off = static_cast<uint64_t *>(get(8 * std::max<uint64_t>(entries, 1)));
   uint64_t *partials = static_cast<uint64_t *>(get(8 * (scan_n / kScanTile + 2)));
   uint64_t *total_d = static_cast<uint64_t *>(get(64));
   SlotDesc *d_slots = static_cast<SlotDesc *>(get(sizeof(SlotDesc) * std::max<size_t>(slots.size(), 1)));
-  if (!ci.rank_pref || !ci.cn_off || !partials || !total_d || !d_slots)
+  RankJob *d_jobs = static_cast<RankJob *>(get(sizeof(RankJob) * nqs));
+  if (!cn_off || !partials || !total_d || !d_slots || !d_jobs)
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate index)");
+  std::vector<RankJob> jobs(nqs);
+  for (uint32_t i = 0; i < nqs; ++i) jobs[i] = {qs[i]->bits, out[i].rank_pref, qs[i]->desc.nq};
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(d_jobs, jobs.data(), sizeof(RankJob) * nqs, hipMemcpyHostToDevice, s));
   kt.begin(CSM_K_AUX);
-  rank_words_kernel<<<grid_for(rank_n), 256, 0, s>>>(q->bits, words, nq, ci.rank_pref);
-  scan_u64_exclusive(ci.rank_pref, rank_n, partials, nullptr, s);
+  rank_words_kernel<<<dim3(std::min<uint32_t>(1024, grid_for(rank_n / nqs + 1)), nqs), 256, 0, s>>>(d_jobs, words);
+  scan_u64_exclusive(rank_all, rank_n, partials, nullptr, s);
   kt.end(CSM_K_AUX, 16ull * rank_n);
   if (slots.empty()) return CSM_OK;
+  for (uint32_t i = 0; i < nqs; ++i) out[i].cn_off = cn_off;
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_slots, slots.data(), sizeof(SlotDesc) * slots.size(), hipMemcpyHostToDevice, s));
-  const dim3 grid(std::max<uint32_t>(1, std::min<uint32_t>(1024, ceil_div(words, kWaves))),
-                  static_cast<uint32_t>(slots.size()));
+  const uint32_t gx = std::max<uint32_t>(1, std::min<uint32_t>(std::max<uint32_t>(8, 1024 / nqs), ceil_div(words, kWaves)));
   uint64_t cand_words = 0;
   for (auto &sd : slots) cand_words += sd.ncand;
-  kt.begin(CSM_K_AUX);
-  cn_kernel<false><<<grid, kThreads, 0, s>>>(g->off, g->adj, words, q->bits, ci.rank_pref, d_slots, ci.cn_off,
-                                             nullptr);
-  scan_u64_exclusive(ci.cn_off, entries, partials, total_d, s);
-  kt.end(CSM_K_AUX, 16ull * words * slots.size() + 16ull * entries);
+  for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {  // grid.y limit
+    const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
+    kt.begin(CSM_K_AUX);
+    cn_kernel<false><<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, nullptr);
+    kt.end(CSM_K_AUX, 16ull * words * ns);
+  }
+  scan_u64_exclusive(cn_off, entries, partials, total_d, s);
   CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, total_d, 8, hipMemcpyDeviceToHost, s));
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-  ci.total_lists = ctx->pinned[0];
-  ci.cn_adj = static_cast<uint32_t *>(get(4 * std::max<uint64_t>(ci.total_lists, 1)));
-  if (!ci.cn_adj) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate lists)");
-  kt.begin(CSM_K_AUX);
-  cn_kernel<true><<<grid, kThreads, 0, s>>>(g->off, g->adj, words, q->bits, ci.rank_pref, d_slots, ci.cn_off,
-                                            ci.cn_adj);
-  kt.end(CSM_K_AUX, 8ull * words * slots.size() + 8ull * cand_words + 4ull * ci.total_lists);
-  ci.build_bytes = 24ull * words * slots.size() + 8ull * cand_words + 16ull * entries + 4ull * ci.total_lists;
+  const uint64_t total = ctx->pinned[0];
+  uint32_t *cn_adj = static_cast<uint32_t *>(get(4 * std::max<uint64_t>(total, 1)));
+  if (!cn_adj) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate lists)");
+  for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {
+    const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
+    kt.begin(CSM_K_AUX);
+    cn_kernel<true><<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, cn_adj);
+    kt.end(CSM_K_AUX, 8ull * words * ns + 4ull * total * ns / slots.size());
+  }
+  for (uint32_t i = 0; i < nqs; ++i) {
+    out[i].cn_adj = cn_adj;
+    out[i].total_lists = total;
+  }
+  out[0].build_bytes = 24ull * words * slots.size() + 8ull * cand_words + 16ull * entries + 4ull * total;
   return CSM_OK;
+}
+
+csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
+                            std::vector<void *> &blocks, KernelTimer &kt) {
+  std::vector<CandIndexHost> out;
+  csm_status st = build_cand_index_batch(ctx, g, {q}, out, blocks, kt);
+  if (st == CSM_OK) ci = out[0];
+  return st;
 }
 
 }  // namespace csm

@@ -56,4 +56,8 @@ struct CandIndexHost {
 csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
                             std::vector<void *> &blocks, KernelTimer &kt);
 
+// All queries' indexes with one launch per pass (one shared list array).
+csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
+                                  std::vector<CandIndexHost> &out, std::vector<void *> &blocks, KernelTimer &kt);
+
 }  // namespace csm

@@ -140,6 +140,7 @@ struct csm_query {
   uint64_t *bits = nullptr;          // nq x words candidate bitmaps (arena)
   csm::QueryDesc *d_desc = nullptr;  // device copy of desc (arena)
   uint32_t *d_cluster_us = nullptr;  // flattened clusters (arena)
+  uint32_t *d_cluster_off = nullptr; // cluster offsets, num_clusters + 1 (arena)
   uint64_t counts[CSM_MAX_QV] = {};  // |C(u)| after the last filter
   bool filtered = false;
   double ms_filter = 0;

@@ -1,4 +1,4 @@
-// Candidate filtering on gfx950.
+// Candidate filtering on gfx950, for one query or a batch (blockIdx.y = query).
 //
 // nlc_filter: one wave per block of 64 consecutive data vertices (lane =
 // vertex).  Low-degree vertices count their neighbour labels serially into a
@@ -7,13 +7,15 @@
 // predicate of filter.cu:486-507 is then evaluated per query vertex and the
 // 64 answers become one bitmap word through a ballot.
 //
-// refine_kernel: the cluster refinement, a "pull" test: for each set bit v of
-// the bitmap being refined, is there a neighbour of v in every one of the
-// given support bitmaps?  Early exit on the first hit.
+// refine_kernel: one step of the cluster refinement, a "pull" test: for each
+// set bit v of the bitmap being refined, is there a neighbour of v in every
+// one of the given support bitmaps?  Early exit on the first hit.  Step s of
+// every query of the batch runs in the same launch.
 #include <algorithm>
 #include <chrono>
 
 #include "common.hpp"
+#include "filter.hpp"
 
 namespace csm {
 
@@ -40,14 +42,15 @@ __device__ inline uint32_t lookup_slot(const FilterSmem &sm, uint32_t l) {
   return kNoSlot;
 }
 
-__global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(
-    const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, const uint32_t *__restrict__ label,
-    uint32_t n, const QueryDesc *__restrict__ gd, uint64_t *__restrict__ bits,
-    unsigned long long *__restrict__ counts, unsigned long long *__restrict__ scanned_deg) {
+__global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(const uint32_t *__restrict__ off,
+                                                                    const uint32_t *__restrict__ adj,
+                                                                    const uint32_t *__restrict__ label, uint32_t n,
+                                                                    const FilterQ *__restrict__ qs) {
   __shared__ FilterSmem sm;
+  const FilterQ fq = qs[blockIdx.y];
   const int tid = threadIdx.x;
   {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(gd);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(fq.desc);
     uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.d);
     for (uint32_t i = tid; i < sizeof(QueryDesc) / 4; i += kFilterThreads) dst[i] = src[i];
     for (uint32_t i = tid; i < kLabelTable; i += kFilterThreads) sm.slot_of[i] = kNoSlot;
@@ -60,6 +63,7 @@ __global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(
 
   const int wave = tid / kWave, lane = tid % kWave;
   const uint32_t words = sm.d.words, nq = sm.d.nq, nslots = sm.d.num_slots;
+  uint64_t *bits = fq.bits;
   uint64_t scanned = 0;
   for (uint32_t b = blockIdx.x * kFilterWaves + wave; b < words; b += gridDim.x * kFilterWaves) {
     const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
@@ -110,20 +114,31 @@ __global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(
     }
   }
   for (int d = kWave / 2; d > 0; d >>= 1) scanned += __shfl_xor(scanned, d, kWave);
-  if (lane == 0 && scanned) atomicAdd(scanned_deg, static_cast<unsigned long long>(scanned));
+  __shared__ unsigned long long s_scan;
+  if (tid == 0) s_scan = 0;
   __syncthreads();
-  if (tid < static_cast<int>(nq) && sm.cnt[tid]) atomicAdd(&counts[tid], sm.cnt[tid]);
+  if (lane == 0 && scanned) atomicAdd(&s_scan, static_cast<unsigned long long>(scanned));
+  __syncthreads();
+  if (tid == 0 && s_scan) atomicAdd(&fq.stats[kStatFilterScanned], s_scan);
+  if (tid < static_cast<int>(nq) && sm.cnt[tid]) atomicAdd(&fq.stats[tid], sm.cnt[tid]);
 }
 
-// CORE: x = cluster core, supports = the members (all must be adjacent).
-// MEMBER: x = member blockIdx.y, support = the core.
+// CORE: x = core of cluster `step`, supports = its members (all must hold).
+// MEMBER: x = member blockIdx.z, support = the core.
 template <bool CORE>
-__global__ __launch_bounds__(kFilterThreads) void refine_kernel(
-    const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n, uint32_t words,
-    uint64_t *__restrict__ bits, const uint32_t *__restrict__ cl_us, uint32_t cb, uint32_t ce,
-    unsigned long long *__restrict__ scanned_deg) {
+__global__ __launch_bounds__(kFilterThreads) void refine_kernel(const uint32_t *__restrict__ off,
+                                                                const uint32_t *__restrict__ adj, uint32_t n,
+                                                                uint32_t words, const FilterQ *__restrict__ qs,
+                                                                uint32_t step) {
+  const FilterQ fq = qs[blockIdx.y];
+  if (step >= fq.num_clusters) return;
+  const uint32_t cb = fq.cl_off[step], ce = fq.cl_off[step + 1];
+  if (ce - cb < 2) return;
+  if (!CORE && blockIdx.z >= ce - cb - 1) return;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const uint32_t x = CORE ? cl_us[cb] : cl_us[cb + 1 + blockIdx.y];
+  const uint32_t *cl_us = fq.cl_us;
+  uint64_t *bits = fq.bits;
+  const uint32_t x = CORE ? cl_us[cb] : cl_us[cb + 1 + blockIdx.z];
   const uint32_t y0 = CORE ? cb + 1 : cb, y1 = CORE ? ce : cb + 1;
   uint64_t *xrow = bits + static_cast<uint64_t>(x) * words;
   uint64_t scanned = 0;
@@ -176,72 +191,126 @@ __global__ __launch_bounds__(kFilterThreads) void refine_kernel(
     if (lane == 0 && nw != xw) xrow[b] = nw;
   }
   for (int d = kWave / 2; d > 0; d >>= 1) scanned += __shfl_xor(scanned, d, kWave);
-  if (lane == 0 && scanned) atomicAdd(scanned_deg, static_cast<unsigned long long>(scanned));
+  __shared__ unsigned long long s_scan;
+  if (threadIdx.x == 0) s_scan = 0;
+  __syncthreads();
+  if (lane == 0 && scanned) atomicAdd(&s_scan, static_cast<unsigned long long>(scanned));
+  __syncthreads();
+  if (threadIdx.x == 0 && s_scan) atomicAdd(&fq.stats[kStatRefineScanned], s_scan);
 }
 
-__global__ __launch_bounds__(kFilterThreads) void count_bits_kernel(const uint64_t *__restrict__ bits, uint32_t words,
-                                                                    unsigned long long *__restrict__ counts) {
-  const uint32_t u = blockIdx.y;
+__global__ __launch_bounds__(kFilterThreads) void count_bits_kernel(uint32_t words, const FilterQ *__restrict__ qs) {
+  const FilterQ fq = qs[blockIdx.y];
+  const uint32_t u = blockIdx.z;
+  if (u >= fq.nq) return;
   unsigned long long c = 0;
   for (uint32_t b = blockIdx.x * kFilterThreads + threadIdx.x; b < words; b += gridDim.x * kFilterThreads)
-    c += __popcll(bits[static_cast<uint64_t>(u) * words + b]);
+    c += __popcll(fq.bits[static_cast<uint64_t>(u) * words + b]);
   for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
-  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(&counts[u], c);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(&fq.stats[u], c);
 }
 
-uint32_t grid_words(uint32_t words, uint32_t per_block) {
-  return std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(words, per_block)));
+// Zero the per-query statistics blocks (a kernel, not a memset: launch
+// sequences may be captured into a graph).
+__global__ void zero_stats_kernel(const FilterQ *__restrict__ qs, uint32_t nqueries, int counts_only) {
+  const uint32_t q = blockIdx.x;
+  if (q >= nqueries) return;
+  const uint32_t n = counts_only ? CSM_MAX_QV : kStatSize;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) qs[q].stats[i] = 0;
+}
+
+uint32_t grid_words(uint32_t words, uint32_t per_block, uint32_t cap) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(cap, ceil_div(words, per_block)));
 }
 
 }  // namespace
+
+void launch_filter_batch(hipStream_t s, const csm_graph *g, const FilterQ *d_qs, uint32_t nqueries, uint32_t words,
+                         const std::vector<uint32_t> &step_members, uint32_t max_nq, bool refine, KernelTimer *kt) {
+  // fewer blocks per query when many queries share the launch
+  const uint32_t cap = std::max<uint32_t>(16, 4096 / std::max<uint32_t>(1, nqueries));
+  const uint32_t gx = grid_words(words, kFilterWaves, cap);
+  zero_stats_kernel<<<nqueries, 64, 0, s>>>(d_qs, nqueries, 0);
+  if (kt) kt->begin(CSM_K_FILTER);
+  nlc_filter_kernel<<<dim3(gx, nqueries), kFilterThreads, 0, s>>>(g->off, g->adj, g->label, g->n, d_qs);
+  if (kt) kt->end(CSM_K_FILTER, 0);
+  if (!refine) return;
+  for (uint32_t c = 0; c < step_members.size(); ++c) {
+    if (step_members[c] == 0) continue;  // no cluster with members at this step
+    if (kt) kt->begin(CSM_K_REFINE);
+    refine_kernel<true><<<dim3(gx, nqueries), kFilterThreads, 0, s>>>(g->off, g->adj, g->n, words, d_qs, c);
+    refine_kernel<false><<<dim3(gx, nqueries, step_members[c]), kFilterThreads, 0, s>>>(g->off, g->adj, g->n, words,
+                                                                                       d_qs, c);
+    if (kt) kt->end(CSM_K_REFINE, 0);
+  }
+  zero_stats_kernel<<<nqueries, 64, 0, s>>>(d_qs, nqueries, 1);
+  if (kt) kt->begin(CSM_K_AUX);
+  count_bits_kernel<<<dim3(grid_words(words, kFilterThreads, cap), nqueries, max_nq), kFilterThreads, 0, s>>>(words,
+                                                                                                           d_qs);
+  if (kt) kt->end(CSM_K_AUX, 0);
+}
+
+std::vector<uint32_t> step_members_of(const std::vector<const csm_query *> &qs) {
+  std::vector<uint32_t> m;
+  for (const csm_query *q : qs) {
+    const csm_query_plan &p = q->plan;
+    if (m.size() < p.num_clusters) m.resize(p.num_clusters, 0);
+    for (uint32_t c = 0; c < p.num_clusters; ++c) {
+      const uint32_t sz = p.cluster_offsets[c + 1] - p.cluster_offsets[c];
+      if (sz >= 2) m[c] = std::max(m[c], sz - 1);
+    }
+  }
+  return m;
+}
+
+FilterQ filter_entry(csm_query *q, uint64_t *stats) {
+  FilterQ f{};
+  f.desc = q->d_desc;
+  f.bits = q->bits;
+  f.cl_us = q->d_cluster_us;
+  f.cl_off = q->d_cluster_off;
+  f.num_clusters = q->plan.num_clusters;
+  f.nq = q->desc.nq;
+  f.stats = reinterpret_cast<unsigned long long *>(stats);
+  return f;
+}
 
 // Filter + refinement for one query on ctx->stream; fills q->counts.
 csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine) {
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
-  const QueryDesc &d = q->desc;
-  const uint32_t words = d.words, nq = d.nq;
-  unsigned long long *dev = static_cast<unsigned long long *>(ctx->arena.alloc(8 * (2 * CSM_MAX_QV + 4)));
-  if (!dev) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (filter counters)");
-  unsigned long long *counts = dev, *scan_filter = dev + CSM_MAX_QV, *scan_refine = dev + CSM_MAX_QV + 1;
-  CSM_HIP_TRY(ctx, hipMemsetAsync(dev, 0, 8 * (2 * CSM_MAX_QV + 4), s));
-  std::memset(q->kstat, 0, sizeof(q->kstat));
-  KernelTimer kt(ctx, q->kstat);
-  kt.begin(CSM_K_FILTER);
-  nlc_filter_kernel<<<grid_words(words, kFilterWaves), kFilterThreads, 0, s>>>(g->off, g->adj, g->label, g->n,
-                                                                              q->d_desc, q->bits, counts,
-                                                                              scan_filter);
-  // B_f = labels + offsets of every vertex + (neighbour id + neighbour label)
-  // for every scanned vertex + the bitmap words; the scanned part is added
-  // below once the device counter is read.
-  kt.end(CSM_K_FILTER, 4ull * g->n + 4ull * (g->n + 1ull) + 8ull * words * nq);
-  const csm_query_plan &p = q->plan;
-  if (refine) {
-    for (uint32_t c = 0; c < p.num_clusters; ++c) {
-      const uint32_t cb = p.cluster_offsets[c], ce = p.cluster_offsets[c + 1];
-      if (ce - cb < 2) continue;
-      kt.begin(CSM_K_REFINE);
-      refine_kernel<true><<<grid_words(words, kFilterWaves), kFilterThreads, 0, s>>>(
-          g->off, g->adj, g->n, words, q->bits, q->d_cluster_us, cb, ce, scan_refine);
-      kt.end(CSM_K_REFINE, 8ull * words * (ce - cb));
-      kt.begin(CSM_K_REFINE);
-      refine_kernel<false><<<dim3(grid_words(words, kFilterWaves), ce - cb - 1), kFilterThreads, 0, s>>>(
-          g->off, g->adj, g->n, words, q->bits, q->d_cluster_us, cb, ce, scan_refine);
-      kt.end(CSM_K_REFINE, 16ull * words * (ce - cb - 1));
-    }
-    CSM_HIP_TRY(ctx, hipMemsetAsync(counts, 0, 8 * CSM_MAX_QV, s));
-    kt.begin(CSM_K_AUX);
-    count_bits_kernel<<<dim3(grid_words(words, kFilterThreads), nq), kFilterThreads, 0, s>>>(q->bits, words, counts);
-    kt.end(CSM_K_AUX, 8ull * words * nq);
+  const uint32_t words = q->desc.words, nq = q->desc.nq;
+  uint64_t *stats = static_cast<uint64_t *>(ctx->arena.alloc(8 * kStatSize));
+  FilterQ *d_q = static_cast<FilterQ *>(ctx->arena.alloc(sizeof(FilterQ)));
+  if (!stats || !d_q) {
+    ctx->arena.free(stats);
+    ctx->arena.free(d_q);
+    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (filter)");
   }
-  unsigned long long host[2 * CSM_MAX_QV + 4];
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(host, dev, sizeof(host), hipMemcpyDeviceToHost, s));
-  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-  kt.flush();
-  q->kstat[CSM_K_FILTER].bytes += 8ull * host[CSM_MAX_QV];
-  q->kstat[CSM_K_REFINE].bytes += 12ull * host[CSM_MAX_QV + 1];  // adjacency word + bitmap word per probe
-  for (uint32_t u = 0; u < nq; ++u) q->counts[u] = host[u];
-  ctx->arena.free(dev);
+  FilterQ fq = filter_entry(q, stats);
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(d_q, &fq, sizeof(FilterQ), hipMemcpyHostToDevice, s));
+  std::memset(q->kstat, 0, sizeof(q->kstat));
+  {
+    KernelTimer kt(ctx, q->kstat);
+    launch_filter_batch(s, g, d_q, 1, words, refine ? step_members_of({q}) : std::vector<uint32_t>(), nq, refine != 0,
+                        &kt);
+    uint64_t host[kStatSize];
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    kt.flush();
+    for (uint32_t u = 0; u < nq; ++u) q->counts[u] = host[u];
+    // algorithmic bytes: labels + offsets of every vertex, (neighbour id +
+    // neighbour label) per scanned adjacency entry, bitmap words written;
+    // refinement: (adjacency id + bitmap word) per probe, bitmap words read
+    q->kstat[CSM_K_FILTER].bytes = 8ull * g->n + 4 + 8ull * host[kStatFilterScanned] + 8ull * words * nq;
+    uint64_t clusters_bytes = 0;
+    for (uint32_t c = 0; c < q->plan.num_clusters; ++c)
+      clusters_bytes += 24ull * words * (q->plan.cluster_offsets[c + 1] - q->plan.cluster_offsets[c]);
+    q->kstat[CSM_K_REFINE].bytes = refine ? 12ull * host[kStatRefineScanned] + clusters_bytes : 0;
+    q->kstat[CSM_K_AUX].bytes += 8ull * words * nq;
+  }
+  ctx->arena.free(stats);
+  ctx->arena.free(d_q);
   q->filtered = true;
   q->ms_filter = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return CSM_OK;

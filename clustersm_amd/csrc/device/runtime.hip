@@ -511,15 +511,19 @@ csm_status csm_query_prepare(csm_ctx *ctx, const csm_graph *g, const csm_query_p
   q->bits = static_cast<uint64_t *>(ctx->arena.alloc(bytes));
   q->d_desc = static_cast<QueryDesc *>(ctx->arena.alloc(sizeof(QueryDesc)));
   q->d_cluster_us = static_cast<uint32_t *>(ctx->arena.alloc(4ull * std::max<uint32_t>(plan->num_cluster_us, 1)));
-  if (!q->bits || !q->d_desc || !q->d_cluster_us) {
+  q->d_cluster_off = static_cast<uint32_t *>(ctx->arena.alloc(4ull * (plan->num_clusters + 1ull)));
+  if (!q->bits || !q->d_desc || !q->d_cluster_us || !q->d_cluster_off) {
     ctx->arena.free(q->bits);
     ctx->arena.free(q->d_desc);
     ctx->arena.free(q->d_cluster_us);
+    ctx->arena.free(q->d_cluster_off);
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (query)");
   }
   hipSetDevice(ctx->device);
   CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_desc, &q->desc, sizeof(QueryDesc), hipMemcpyHostToDevice, ctx->stream));
   CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_cluster_us, plan->cluster_us, 4ull * plan->num_cluster_us,
+                                  hipMemcpyHostToDevice, ctx->stream));
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_cluster_off, plan->cluster_offsets, 4ull * (plan->num_clusters + 1ull),
                                   hipMemcpyHostToDevice, ctx->stream));
   CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   *out = q.release();
@@ -532,6 +536,7 @@ csm_status csm_query_free(csm_ctx *ctx, csm_query *q) {
   ctx->arena.free(q->bits);
   ctx->arena.free(q->d_desc);
   ctx->arena.free(q->d_cluster_us);
+  ctx->arena.free(q->d_cluster_off);
   delete q;
   return CSM_OK;
 }

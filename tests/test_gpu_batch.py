@@ -1,0 +1,91 @@
+"""Batched path (one persistent kernel for many queries): counts and cap
+flags identical to the CPU oracle and to the plain per-query path, alone and
+in the mix, with and without graph capture, and under pool pressure."""
+import random
+
+import numpy as np
+import pytest
+
+import clustersm_amd as csm
+from helpers import rand_graph, rand_query
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = csm.Context(0, 16 << 30)
+    yield c
+    c.close()
+
+
+def _oracle_counts(dl, de, queries, cap):
+    d = O.Data.from_arrays(dl, de)
+    out = []
+    for ql, qe in queries:
+        oq = O.Query.from_arrays(ql, qe)
+        c, capped, _, _, _ = O.match(d, oq, oq.plan(), cap=cap if cap else (1 << 64) - 1, threads=8)
+        out.append((c, capped))
+    return out
+
+
+def test_batch_small_random_vs_oracle(ctx):
+    r = random.Random(31)
+    for round_ in range(6):
+        n = r.randint(40, 160)
+        nl = r.randint(1, 3)
+        dl, de = rand_graph(r, n, r.uniform(0.03, 0.15), nl)
+        queries = [rand_query(r, r.randint(2, 7), r.randint(0, 3), nl) for _ in range(r.randint(1, 24))]
+        g = ctx.upload_edges(dl, de)
+        qs = [g.prepare(csm.HostGraph.from_edges(ql, qe).plan()) for ql, qe in queries]
+        for cap in (0, 50):
+            want = _oracle_counts(dl, de, queries, cap)
+            for use_graph in (True, False):
+                print("batch round", round_, "queries", len(qs), "cap", cap, "graph", use_graph, flush=True)
+                res = csm.batch_match(g, qs, cap=cap, use_graph=use_graph)
+                got = [(x.count, bool(x.capped)) for x in res]
+                assert got == want, (round_, cap, use_graph)
+        for q in qs:
+            q.free()
+        g.free()
+
+
+def test_batch_equals_plain_and_alone(ctx):
+    """Each query's result is the same alone (plain path, batch of one) and in the mix."""
+    hg = csm.gen_rmat(12, 40000, 8, 1.0, 9)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    r = random.Random(3)
+    plans = []
+    for i in range(24):
+        nv = r.randint(3, 12)
+        plans.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 500 + i, avg_degree=r.uniform(2, 3.5)).plan())
+    qs = [g.prepare(p) for p in plans]
+    cap = 200_000
+    mix = csm.batch_match(g, qs, cap=cap)
+    for i, q in enumerate(qs):
+        plain, _ = q.match(cap=cap)
+        alone = csm.batch_match(g, [q], cap=cap)[0]
+        print("query", i, plans[i].num_v, plain.count, mix[i].count, flush=True)
+        assert (mix[i].count, mix[i].capped) == (plain.count, plain.capped) == (alone.count, alone.capped), i
+    for q in qs:
+        q.free()
+    g.free()
+
+
+def test_batch_small_pool_continuations(ctx):
+    """A pool of a few blocks forces yields and, if it runs dry, the plain fallback."""
+    hg = csm.gen_rmat(11, 30000, 4, 0.5, 4)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    plans = [csm.gen_query(host, 6, csm.QUERY_TREE, 77 + i).plan() for i in range(6)]
+    qs = [g.prepare(p) for p in plans]
+    cap = 3_000_000
+    want = [q.match(cap=cap)[0] for q in qs]
+    for pool in (64 * 128 * 1024, 256 * 128 * 1024):
+        res = csm.batch_match(g, qs, cap=cap, pool_bytes=pool)
+        assert [(x.count, x.capped) for x in res] == [(w.count, w.capped) for w in want], pool
+    for q in qs:
+        q.free()
+    g.free()
