@@ -29,6 +29,8 @@
 // Counts only: a capped list needs the plain path's ordered enumeration.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 
 #include "cand_index.hpp"
@@ -60,23 +62,33 @@ struct BQuery {
 
 struct BState {
   unsigned long long count;
-  unsigned int done;
-  unsigned int pending;  // tasks of the query pushed and not yet finished
+  unsigned int done;         // kRunning, kCapped or kAbandoned
+  unsigned int pad;
+  unsigned long long items;  // (row, list entry) pairs examined for the query
+  unsigned long long rows;   // partial rows written for the query
 };
 
-// One record per pool block (a block's rows form at most one task) and one
-// per root range; `next` links the records of a query's stack.
+enum : unsigned int { kRunning = 0, kCapped = 1, kAbandoned = 2 };
+
+// A task record: one per pool block (a block's rows form at most one task)
+// and one per root range.
 struct Task {
-  uint32_t q, k, block, r0, r1, skip_rows, skip_items, next;
-};
-
-struct QStack {
-  unsigned int lock;
-  unsigned int top;
+  uint32_t q, k, block, r0, r1, skip_rows, skip_items, pad;
 };
 
 constexpr uint32_t kNil = 0xFFFFFFFFu;
-constexpr uint32_t kMaxHeld = 4;  // full output blocks a task keeps before yielding
+constexpr uint32_t kDequeCap = 1024;  // > levels x (held blocks + 2): a deque never fills
+
+// One task deque per workgroup.  The owner pushes and pops at `bottom`
+// (depth-first, lexicographically first rows on top); other workgroups
+// steal the oldest task at `top`.  A short lock guards both ends.
+struct Deque {
+  unsigned int lock;
+  unsigned int pad;
+  unsigned long long top;
+  unsigned long long bottom;
+  unsigned long long pad2;
+};
 
 struct FreeSlot {
   uint32_t id, pad;
@@ -84,77 +96,127 @@ struct FreeSlot {
 };
 
 struct BCtl {
-  unsigned long long outstanding;
-  unsigned long long alloc_fail;
+  unsigned long long root_next;    // next root task to hand out
+  unsigned long long idle;         // workgroups without a task
   unsigned long long items;        // (row, list entry) pairs examined
   unsigned long long rows_in;      // rows read
   unsigned long long rows_out;     // rows written
   unsigned long long tasks;
+  unsigned long long steals;
+  unsigned long long fails;        // failed block allocations
+  unsigned long long busy_ticks;   // real-time ticks inside tasks, summed over workgroups
+  unsigned long long idle_ticks;   // real-time ticks without a task
+  unsigned long long sync_ticks;   // real-time ticks in deque and free-ring operations
+  unsigned long long alloc_fail;   // consecutive failed allocations from the free ring
   unsigned int abort;
-  unsigned int pad;
+  unsigned int overflow;           // a deque was full (never expected)
 };
 
 struct BArgs {
-  const uint32_t *off, *adj;
   AdjProbe probe;
   const BQuery *qs;
   BState *st;
   BCtl *ctl;
   Task *recs;                      // num_blocks + root tasks
-  QStack *stacks;                  // one per query
+  Deque *dq;                       // one per workgroup
+  uint32_t *dq_buf;                // kDequeCap record ids per workgroup
   uint32_t nqueries;
+  uint32_t tasks_per_query;        // root tasks per query
+  uint32_t words;                  // bitmap words
+  uint32_t root_words;             // root bitmap words per root task
   FreeSlot *free_ring;
   unsigned long long *free_head, *free_tail;
   uint32_t *pool;
   uint64_t num_blocks;
   uint64_t block_words;            // u32 words per block
-  uint32_t root_words;             // root bitmap words per level-1 task
-  uint32_t max_k;                  // widest row (columns) in LDS
+  uint32_t ilp;                    // items per thread per tile (1..kIlp)
+  uint32_t max_out;                // full output blocks a task holds before it yields
+  uint64_t budget;                 // items per query before it is handed to the plain path
+  uint32_t cache_cap;              // free blocks a workgroup may keep (<= kCache)
   uint64_t deadline_ticks;         // s_memrealtime ticks (100 MHz) before the kernel gives up
 };
 
 __device__ inline unsigned long long ld_rlx(unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline void st_rlx(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ inline void st_rel(unsigned long long *p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t ld_done(const BArgs &a, uint32_t q) {
+  return __hip_atomic_load(&a.st[q].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool aborted(const BArgs &a) {
+  return __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 __device__ inline void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 constexpr uint32_t kMaxSpin = 1u << 24;
 
-__device__ inline uint32_t ld_rlx32(unsigned int *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Per-query lock: held by one thread for a few loads and stores, never
-// across a wait, so the spin always ends.
-__device__ inline void qlock(unsigned int *l) {
-  while (atomicCAS(l, 0u, 1u) != 0u) __builtin_amdgcn_s_sleep(1);
+// Deque lock: held by one thread for a few loads and stores, never across a
+// wait, and contended only by the owner and an occasional thief.
+__device__ inline void dlock(unsigned int *l) {
+  while (atomicCAS(l, 0u, 1u) != 0u) __builtin_amdgcn_s_sleep(2);
   acquire_agent();
 }
-__device__ inline void qunlock(unsigned int *l) {
+__device__ inline void dunlock(unsigned int *l) {
   __hip_atomic_store(l, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Record fields must be written before the push (the unlock publishes them).
-__device__ void stack_push(const BArgs &a, uint32_t q, uint32_t rec) {
-  QStack *st = &a.stacks[q];
-  qlock(&st->lock);
-  a.recs[rec].next = ld_rlx32(&st->top);
-  __hip_atomic_store(&st->top, rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  qunlock(&st->lock);
+// Owner: push n records in order (the last one ends on top).  Record
+// fields and their rows are written before; the unlock publishes them.
+__device__ void deque_push(const BArgs &a, uint32_t wg, const uint32_t *recs, uint32_t n) {
+  if (n == 0) return;
+  Deque *d = &a.dq[wg];
+  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
+  dlock(&d->lock);
+  const unsigned long long t = ld_rlx(&d->top), b = ld_rlx(&d->bottom);
+  if (b - t + n > kDequeCap) {
+    atomicExch(&a.ctl->overflow, 1u);
+    atomicExch(&a.ctl->abort, 1u);
+  } else {
+    for (uint32_t i = 0; i < n; ++i) buf[(b + i) % kDequeCap] = recs[i];
+    st_rlx(&d->bottom, b + n);
+  }
+  dunlock(&d->lock);
 }
 
-__device__ bool stack_pop(const BArgs &a, uint32_t q, uint32_t &rec) {
-  QStack *st = &a.stacks[q];
-  if (ld_rlx32(&st->top) == kNil) return false;
-  qlock(&st->lock);
-  const uint32_t t = ld_rlx32(&st->top);
-  if (t != kNil) __hip_atomic_store(&st->top, a.recs[t].next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  qunlock(&st->lock);
-  rec = t;
-  return t != kNil;
+// Owner: newest record, or kNil.
+__device__ uint32_t deque_pop(const BArgs &a, uint32_t wg) {
+  Deque *d = &a.dq[wg];
+  if (ld_rlx(&d->bottom) == ld_rlx(&d->top)) return kNil;
+  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
+  dlock(&d->lock);
+  const unsigned long long t = ld_rlx(&d->top), b = ld_rlx(&d->bottom);
+  uint32_t r = kNil;
+  if (b > t) {
+    r = buf[(b - 1) % kDequeCap];
+    st_rlx(&d->bottom, b - 1);
+  }
+  dunlock(&d->lock);
+  return r;
+}
+
+// Thief: oldest record of workgroup v, or kNil.  A thief counted idle
+// leaves the idle count while it holds v's lock: v can only declare itself
+// idle after the unlock, so the count never reaches every workgroup while a
+// task is in hand.
+__device__ uint32_t deque_steal(const BArgs &a, uint32_t v, bool counted_idle) {
+  Deque *d = &a.dq[v];
+  if (ld_rlx(&d->bottom) <= ld_rlx(&d->top)) return kNil;
+  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(v) * kDequeCap;
+  dlock(&d->lock);
+  const unsigned long long t = ld_rlx(&d->top), b = ld_rlx(&d->bottom);
+  uint32_t r = kNil;
+  if (b > t) {
+    r = buf[t % kDequeCap];
+    st_rlx(&d->top, t + 1);
+    if (counted_idle) atomicAdd(&a.ctl->idle, ~0ull);
+  }
+  dunlock(&d->lock);
+  return r;
 }
 
 __device__ void free_push(const BArgs &a, uint32_t id) {  // bounded MPMC ring, seq 2t free / 2t+1 full
@@ -188,61 +250,38 @@ __device__ bool free_pop(const BArgs &a, uint32_t &id) {
   }
 }
 
-__device__ inline void push_task(const BArgs &a, uint32_t rec, const Task &t) {
-  Task &r = a.recs[rec];
-  r.q = t.q;
-  r.k = t.k;
-  r.block = t.block;
-  r.r0 = t.r0;
-  r.r1 = t.r1;
-  r.skip_rows = t.skip_rows;
-  r.skip_items = t.skip_items;
-  atomicAdd(&a.ctl->outstanding, 1ull);
-  atomicAdd(&a.st[t.q].pending, 1u);
-  stack_push(a, t.q, rec);
-}
-
-// Initial state: every block in the free ring, each query's root tasks
-// linked in ascending order on its stack, counters zero.
-__global__ void batch_init_kernel(BArgs a, uint32_t tasks_per_query, uint32_t words) {
+// Initial state: every block in the free ring, deques empty, counters zero.
+__global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x, stride = 256ull * gridDim.x;
-  const uint32_t nqs = a.nqueries;
-  const uint64_t nroot = static_cast<uint64_t>(nqs) * tasks_per_query;
-  for (uint64_t i = tid; i < nroot; i += stride) {
-    const uint32_t q = static_cast<uint32_t>(i / tasks_per_query), c = static_cast<uint32_t>(i % tasks_per_query);
-    Task &t = a.recs[a.num_blocks + i];
-    t.q = q;
-    t.k = 1;
-    t.block = 0;
-    t.r0 = c * a.root_words;
-    t.r1 = min(words, (c + 1) * a.root_words);
-    t.skip_rows = 0;
-    t.skip_items = 0;
-    t.next = c + 1 < tasks_per_query ? static_cast<uint32_t>(a.num_blocks + i + 1) : kNil;
-  }
   for (uint64_t i = tid; i < a.num_blocks; i += stride) {
     a.free_ring[i].id = static_cast<uint32_t>(i);
     a.free_ring[i].seq = 2 * i + 1;
   }
-  for (uint64_t i = tid; i < nqs; i += stride) {
-    const bool act = a.qs[i].active != 0;
+  for (uint64_t i = tid; i < nwgs; i += stride) {
+    a.dq[i].lock = 0;
+    a.dq[i].top = 0;
+    a.dq[i].bottom = 0;
+  }
+  for (uint64_t i = tid; i < a.nqueries; i += stride) {
     a.st[i].count = 0;
-    a.st[i].done = act ? 0u : 1u;
-    a.st[i].pending = act ? tasks_per_query : 0u;
-    a.stacks[i].lock = 0;
-    a.stacks[i].top = act ? static_cast<uint32_t>(a.num_blocks + i * tasks_per_query) : kNil;
+    a.st[i].done = a.qs[i].active ? kRunning : kCapped;
+    a.st[i].items = a.st[i].rows = 0;
   }
   if (tid == 0) {
-    uint64_t live = 0;
-    for (uint32_t q = 0; q < nqs; ++q) live += a.qs[q].active ? tasks_per_query : 0;
     *a.free_head = 0;
     *a.free_tail = a.num_blocks;
-    a.ctl->outstanding = live;
-    a.ctl->alloc_fail = 0;
-    a.ctl->items = a.ctl->rows_in = a.ctl->rows_out = a.ctl->tasks = 0;
-    a.ctl->abort = 0;
+    BCtl &c = *a.ctl;
+    c.root_next = c.idle = c.items = c.rows_in = c.rows_out = c.tasks = c.steals = c.fails = 0;
+    c.busy_ticks = c.idle_ticks = c.sync_ticks = 0;
+    c.alloc_fail = 0;
+    c.abort = c.overflow = 0;
   }
 }
+
+constexpr uint32_t kIlp = 4;     // items per thread per tile, checked in lockstep
+constexpr uint32_t kMaxA = 4;    // anchors whose candidate lists are kept per row
+constexpr uint32_t kMaxOut = 8;  // most full output blocks a task holds before it yields
+constexpr uint32_t kCache = 16;  // free blocks a workgroup keeps for itself
 
 struct Smem {
   Task task;
@@ -250,23 +289,73 @@ struct Smem {
   LevelJoin lv;
   LevelJoin lv_last;
   uint32_t roots[kMaxRootWords * 64];
-  uint32_t nroots;
   uint64_t pre[kRowBatch + 1];
-  uint64_t beg[kRowBatch];
+  uint64_t abeg[kRowBatch][kMaxA];  // candidate list of each kept anchor: begin
+  uint32_t alen[kRowBatch][kMaxA];  // and length
+  uint64_t dbeg[kRowBatch];         // driver list start (after a resume skip)
   uint8_t drv[kRowBatch];
-  uint32_t wcnt[kWaves];
   unsigned long long red[kWaves];
+  uint32_t tile_n[3];               // rotating survivor counters, one per tile in flight
   uint32_t out_block;
-  uint32_t out_fill;
-  uint32_t held[kMaxHeld];         // full output blocks of this task, in row order
-  uint32_t held_fill[kMaxHeld];
-  uint32_t nheld;
-  uint32_t rec;                    // record of the task being processed
-  uint32_t cur_q;
-  int flag;      // 0 go on, 1 yield with continuation
-  int state;     // loop control: 0 work, 1 idle, 2 exit
-  int stop;      // broadcast of the query's done flag
+  uint32_t rec;                     // record of the task being processed
+  uint32_t staged_q, staged_k;      // query header and level descriptor held in LDS
+  uint32_t nheld;                   // full output blocks of the current task, in row order
+  uint32_t held[kMaxOut];
+  uint32_t held_fill[kMaxOut];
+  uint32_t push[kMaxOut + 2];       // records pushed when the task ends
+  uint32_t cache[kCache];           // free blocks owned by this workgroup
+  uint32_t ncache;
+  // counts and statistics not yet added to the query's totals
+  uint32_t acc_q;
+  unsigned long long acc_count, acc_items, acc_rows;
+  unsigned long long sync_ticks;
+  int flag;                         // 0 go on, 1 output bound reached, 2 pool empty: yield with continuation
+  int state;                        // loop control: 0 work, 1 idle, 2 exit
+  int stop;                         // broadcast: the task's query is finished or the kernel aborts
 };
+
+// Thread 0.  Blocks come from the workgroup's cache first.
+__device__ bool blk_alloc(const BArgs &a, Smem &sm, uint32_t &id) {
+  if (sm.ncache) {
+    id = sm.cache[--sm.ncache];
+    return true;
+  }
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  const bool ok = free_pop(a, id);
+  if (ok) {
+    if (ld_rlx(&a.ctl->alloc_fail)) atomicExch(&a.ctl->alloc_fail, 0ull);
+  } else if (atomicAdd(&a.ctl->alloc_fail, 1ull) > kAbortAfterFails) {
+    atomicExch(&a.ctl->abort, 1u);  // no block freed for a long time: the host finishes the batch
+  }
+  sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
+  return ok;
+}
+__device__ void blk_free(const BArgs &a, Smem &sm, uint32_t id) {
+  if (sm.ncache < a.cache_cap) {
+    sm.cache[sm.ncache++] = id;
+    return;
+  }
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  free_push(a, id);
+  sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
+}
+
+// Thread 0: adds the workgroup's pending count and statistics to query
+// acc_q and marks the query capped or over its budget.
+__device__ void flush_acc(const BArgs &a, Smem &sm) {
+  const uint32_t q = sm.acc_q;
+  if (q == kNil) return;
+  if (sm.acc_count) {
+    const unsigned long long c = atomicAdd(&a.st[q].count, sm.acc_count) + sm.acc_count;
+    if (c > a.qs[q].cap) atomicExch(&a.st[q].done, kCapped);
+  }
+  if (sm.acc_items) {
+    const unsigned long long it = atomicAdd(&a.st[q].items, sm.acc_items) + sm.acc_items;
+    if (it > a.budget) atomicCAS(&a.st[q].done, kRunning, kAbandoned);
+  }
+  if (sm.acc_rows) atomicAdd(&a.st[q].rows, sm.acc_rows);
+  sm.acc_count = sm.acc_items = sm.acc_rows = 0;
+}
 
 template <typename T>
 __device__ inline T block_reduce(T v, T *lds) {
@@ -300,22 +389,43 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t *lds, uint64_t &
 
 __device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
 
+__device__ inline uint32_t row_of(const Smem &sm, uint32_t nb, uint64_t it) {
+  uint32_t lo = 0, hi = nb;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sm.pre[mid] <= it) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline void write_rec(const BArgs &a, uint32_t rec, const Task &t) {
+  Task &r = a.recs[rec];
+  r.q = t.q;
+  r.k = t.k;
+  r.block = t.block;
+  r.r0 = t.r0;
+  r.r1 = t.r1;
+  r.skip_rows = t.skip_rows;
+  r.skip_items = t.skip_items;
+}
+
 // Process one task; returns with all threads converged.
-__device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows) {
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+__device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_t wg, unsigned long long &n_items,
+                             unsigned long long &n_rows_in, unsigned long long &n_rows_out) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const Task t = sm.task;
   const BQuery &Q = sm.q;
   const uint32_t k = t.k;
   const bool count_level = k == Q.stop;
   const uint32_t kout = k + 1;
   const uint32_t out_cap = static_cast<uint32_t>(a.block_words / kout);
+  const uint32_t ilp = count_level ? kIlp : min(a.ilp, out_cap / kThreads);
+  const uint32_t nanchor = sm.lv.nanchor;
   // rows of the task
   uint32_t nrows;
   const uint32_t *grows = nullptr;
   if (k == 1) {
-    if (tid == 0) sm.nroots = 0;
-    __syncthreads();
-    // each wave takes words; set bits appended in order through a scan
+    // set bits of the root words, appended in order through a scan
     const uint32_t nw = t.r1 - t.r0;
     uint64_t w = 0;
     uint32_t c = 0;
@@ -337,36 +447,39 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows) {
     nrows = t.r1 - t.r0;
     grows = a.pool + static_cast<uint64_t>(t.block) * a.block_words + static_cast<uint64_t>(t.r0) * k;
   }
-  unsigned long long contrib = 0;
-  unsigned long long items_seen = 0, rows_seen = 0, rows_written = 0;
+  unsigned long long contrib = 0, items_seen = 0, rows_written = 0;
   uint32_t row0 = t.skip_rows;
   uint32_t skip_items = t.skip_items;
+  uint32_t fill = 0;  // rows in the current output block (same value in every thread)
+  uint32_t tiles = 0;
   if (tid == 0) {
     sm.flag = 0;
     sm.nheld = 0;
-    sm.out_fill = 0;
     sm.out_block = kNil;
+    sm.tile_n[0] = sm.tile_n[1] = sm.tile_n[2] = 0;
   }
   __syncthreads();
   for (; row0 < nrows; row0 += kRowBatch) {
-    if (tid == 0)
-      sm.stop = __hip_atomic_load(&a.st[t.q].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    if (tid == 0) sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
     __syncthreads();
     if (sm.stop) break;
     const uint32_t nb = min(kRowBatch, nrows - row0);
-    // stage rows and choose each row's driver list
     for (uint32_t i = tid; i < nb * k; i += kThreads)
       s_rows[i] = k == 1 ? sm.roots[row0 + i] : grows[static_cast<uint64_t>(row0) * k + i];
     __syncthreads();
+    // each row: candidate lists of its anchors; the shortest drives
     uint64_t len = 0;
     if (static_cast<uint32_t>(tid) < nb) {
       const uint32_t *row = s_rows + tid * k;
       uint64_t best = ~0ull, bbeg = 0;
       uint32_t bi = 0;
-      for (uint32_t x = 0; x < sm.lv.nanchor; ++x) {
+      for (uint32_t x = 0; x < nanchor; ++x) {
         uint64_t b0, e0;
         Q.ci.list(sm.lv, x, row[sm.lv.anchor[x]], b0, e0);
+        if (x < kMaxA) {
+          sm.abeg[tid][x] = b0;
+          sm.alen[tid][x] = static_cast<uint32_t>(e0 - b0);
+        }
         if (e0 - b0 < best) {
           best = e0 - b0;
           bbeg = b0;
@@ -377,7 +490,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows) {
         bbeg += skip_items;
         best -= skip_items;
       }
-      sm.beg[tid] = bbeg;
+      sm.dbeg[tid] = bbeg;
       sm.drv[tid] = static_cast<uint8_t>(bi);
       len = best;
     }
@@ -388,214 +501,343 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows) {
     if (static_cast<uint32_t>(tid) < nb) sm.pre[tid] = ex;
     if (tid == 0) sm.pre[nb] = total;
     __syncthreads();
-    rows_seen += nb;
-    for (uint64_t base = 0; base < total; base += kThreads) {
-      const uint64_t it = base + tid;
-      bool ok = false;
-      uint32_t w = 0, ri = 0;
-      if (it < total) {
-        uint32_t lo = 0, hi = nb;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (sm.pre[mid] <= it) lo = mid; else hi = mid;
+    n_rows_in += nb;
+    const uint64_t step = static_cast<uint64_t>(kThreads) * ilp;
+    for (uint64_t base = 0; base < total; base += step) {
+      uint32_t w[kIlp], ri[kIlp];
+      bool ok[kIlp];
+#pragma unroll
+      for (uint32_t j = 0; j < kIlp; ++j) {
+        const uint64_t it = base + j * kThreads + tid;
+        ok[j] = j < ilp && it < total;
+        ri[j] = 0;
+        w[j] = 0;
+        if (ok[j]) {
+          ri[j] = row_of(sm, nb, it);
+          w[j] = Q.ci.cn_adj[sm.dbeg[ri[j]] + (it - sm.pre[ri[j]])];
         }
-        ri = lo;
-        const uint32_t *row = s_rows + ri * k;
-        const uint32_t d = sm.drv[ri];
-        w = Q.ci.cn_adj[sm.beg[ri] + (it - sm.pre[ri])];
-        ok = true;
-        for (uint32_t x = 0; ok && x < sm.lv.ninj; ++x) ok = row[sm.lv.inj[x]] != w;
-        for (uint32_t x = 0; ok && x < sm.lv.nanchor; ++x)
-          if (x != d) ok = a.probe.has(row[sm.lv.anchor[x]], w);
-        if (count_level && ok) {
-          if (Q.fused) {
-            const uint32_t pk = sm.lv_last.parent;
-            const uint32_t xv = pk == k ? w : row[pk];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kIlp; ++j) items_seen += ok[j] ? 1 : 0;
+      // injectivity against rows in LDS
+      for (uint32_t x = 0; x < sm.lv.ninj; ++x) {
+        const uint32_t col = sm.lv.inj[x];
+#pragma unroll
+        for (uint32_t j = 0; j < kIlp; ++j)
+          if (ok[j]) ok[j] = s_rows[ri[j] * k + col] != w[j];
+      }
+      // adjacency to the other anchors: w is in C(u_k), so w ~ r[a] iff w is
+      // in the anchor's candidate list; binary searches run in lockstep
+      for (uint32_t x = 0; x < nanchor; ++x) {
+        if (x < kMaxA) {
+          uint64_t lo[kIlp];
+          uint32_t n[kIlp];
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            const bool act = ok[j] && sm.drv[ri[j]] != x;
+            lo[j] = sm.abeg[ri[j]][x];
+            n[j] = act ? sm.alen[ri[j]][x] : 0u;
+          }
+          for (;;) {
+            bool any = false;
+#pragma unroll
+            for (uint32_t j = 0; j < kIlp; ++j) any |= n[j] > 1;
+            if (!any) break;
+#pragma unroll
+            for (uint32_t j = 0; j < kIlp; ++j) {
+              if (n[j] > 1) {
+                const uint32_t half = n[j] >> 1;
+                const uint32_t v = Q.ci.cn_adj[lo[j] + half];
+                if (v <= w[j]) lo[j] += half;
+                n[j] -= half;
+              }
+            }
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            if (ok[j] && sm.drv[ri[j]] != x)
+              ok[j] = sm.alen[ri[j]][x] != 0 && Q.ci.cn_adj[lo[j]] == w[j];
+          }
+        } else {
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j)
+            if (ok[j] && sm.drv[ri[j]] != x) ok[j] = a.probe.has(s_rows[ri[j] * k + sm.lv.anchor[x]], w[j]);
+        }
+      }
+      if (count_level) {
+        if (Q.fused) {
+          const uint32_t pk = sm.lv_last.parent;
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            if (!ok[j]) continue;
+            const uint32_t *row = s_rows + ri[j] * k;
+            const uint32_t xv = pk == k ? w[j] : row[pk];
             uint64_t lb, le;
             Q.ci.list(sm.lv_last, 0, xv, lb, le);
             long long c = static_cast<long long>(le - lb);
             for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) {
-              const uint32_t j = sm.lv_last.inj[x];
-              const uint32_t v = j == k ? w : row[j];
+              const uint32_t col = sm.lv_last.inj[x];
+              const uint32_t v = col == k ? w[j] : row[col];
               if (v != xv && bit_of(Q.cand_last, v) && a.probe.has(xv, v)) --c;
             }
             contrib += static_cast<unsigned long long>(c);
-          } else {
-            contrib += 1;
           }
+        } else {
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) contrib += ok[j] ? 1 : 0;
         }
-      }
-      if (count_level) {
-        items_seen += it < total ? 1 : 0;
         continue;
       }
-      // survivors go to the current output block, in item order
-      const uint64_t m = ballot(ok);
-      if (lane == 0) sm.wcnt[wave] = static_cast<uint32_t>(__popcll(m));
-      __syncthreads();
-      uint32_t before = 0, tile = 0;
-      for (int x = 0; x < kWaves; ++x) {
-        if (x < wave) before += sm.wcnt[x];
-        tile += sm.wcnt[x];
+      // survivors: a position per wave from an LDS counter, then per lane
+      uint64_t m[kIlp];
+      uint32_t wcnt = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < kIlp; ++j) {
+        m[j] = ballot(ok[j]);
+        wcnt += static_cast<uint32_t>(__popcll(m[j]));
       }
-      if (tid == 0 && tile && (sm.out_block == kNil || sm.out_fill + tile > out_cap)) {
-        if (sm.out_block != kNil) {  // the current block is full: keep it
-          sm.held[sm.nheld++] = sm.out_block | 0u;
-          sm.held_fill[sm.nheld - 1] = sm.out_fill;
-          sm.out_block = kNil;
-        }
-        uint32_t id;
-        if (sm.nheld == kMaxHeld) {
-          sm.flag = 1;  // bounded output per task: yield before this tile
-        } else if (free_pop(a, id)) {
-          sm.out_block = id;
-          sm.out_fill = 0;
-          atomicExch(&a.ctl->alloc_fail, 0ull);
-        } else {
-          sm.flag = 2;  // pool empty: yield before this tile, move to another query
-          if (atomicAdd(&a.ctl->alloc_fail, 1ull) > kAbortAfterFails) atomicExch(&a.ctl->abort, 1u);
-        }
-        if (sm.flag) {  // resume point: the row holding item `base`
-          uint32_t lo = 0, hi = nb;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (sm.pre[mid] <= base) lo = mid; else hi = mid;
+      const uint32_t cslot = tiles % 3;
+      uint32_t wpos = 0;
+      if (lane == 0 && wcnt) wpos = atomicAdd(&sm.tile_n[cslot], wcnt);
+      wpos = __shfl(wpos, 0, kWave);
+      if (tid == 0) sm.tile_n[(tiles + 1) % 3] = 0;  // next tile's counter, last read two tiles ago
+      __syncthreads();
+      const uint32_t tile = sm.tile_n[cslot];
+      ++tiles;
+      if (tile && (sm.out_block == kNil || fill + tile > out_cap)) {  // uniform
+        __syncthreads();
+        if (tid == 0) {
+          if (sm.out_block != kNil) {  // keep the full block; it is pushed when the task ends or yields
+            sm.held[sm.nheld] = sm.out_block;
+            sm.held_fill[sm.nheld] = fill;
+            sm.out_block = kNil;
+            if (++sm.nheld >= a.max_out) sm.flag = 1;  // bounded output: deeper work goes first
           }
-          sm.task.skip_rows = row0 + lo;
-          sm.task.skip_items = static_cast<uint32_t>(base - sm.pre[lo] + (lo == 0 ? first_skip : 0));
+          uint32_t id;
+          if (sm.flag) {
+          } else if (blk_alloc(a, sm, id)) {
+            sm.out_block = id;
+          } else {
+            sm.flag = 2;  // pool empty: yield before this tile
+            atomicAdd(&a.ctl->fails, 1ull);
+          }
+          if (sm.flag) {  // resume at the row holding item `base`
+            const uint32_t lo = row_of(sm, nb, base);
+            sm.task.skip_rows = row0 + lo;
+            sm.task.skip_items = static_cast<uint32_t>(base - sm.pre[lo] + (lo == 0 ? first_skip : 0));
+          }
+        }
+        __syncthreads();
+        fill = 0;
+        if (sm.flag) break;
+      }
+      if (wcnt) {
+        uint32_t *blk = a.pool + static_cast<uint64_t>(sm.out_block) * a.block_words;
+        uint32_t pos = fill + wpos;
+#pragma unroll
+        for (uint32_t j = 0; j < kIlp; ++j) {
+          if (ok[j]) {
+            uint32_t *dst = blk + static_cast<uint64_t>(pos + lanes_below(m[j])) * kout;
+            const uint32_t *row = s_rows + ri[j] * k;
+            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
+            dst[k] = w[j];
+          }
+          pos += static_cast<uint32_t>(__popcll(m[j]));
         }
       }
-      __syncthreads();
-      if (sm.flag) break;
-      items_seen += it < total ? 1 : 0;
-      if (ok) {
-        uint32_t *dst = a.pool + static_cast<uint64_t>(sm.out_block) * a.block_words +
-                        static_cast<uint64_t>(sm.out_fill + before + lanes_below(m)) * kout;
-        const uint32_t *row = s_rows + ri * k;
-        for (uint32_t j = 0; j < k; ++j) dst[j] = row[j];
-        dst[k] = w;
-      }
-      rows_written += ok ? 1 : 0;
-      __syncthreads();
-      if (tid == 0) sm.out_fill += tile;
-      __syncthreads();
+      rows_written += tid == 0 ? tile : 0;
+      fill += tile;
     }
     if (sm.flag) break;
     __syncthreads();
   }
-  // finish: publish rows, push continuation below the new blocks, count, free input
+  // finish: publish rows, push continuation and blocks, free the input
   __threadfence();
   const unsigned long long tot_contrib = block_reduce(contrib, sm.red);
   const unsigned long long tot_items = block_reduce(items_seen, sm.red);
-  const unsigned long long tot_written = block_reduce(rows_written, sm.red);
   if (tid == 0) {
-    __threadfence();
-    const bool done = __hip_atomic_load(&a.st[t.q].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    const bool cont = sm.flag != 0 && !done;
-    if (cont) {
-      push_task(a, sm.rec, sm.task);  // same rows, resume point updated
+    const bool live = ld_done(a, t.q) == kRunning && !aborted(a);
+    uint32_t np = 0;
+    if (sm.flag != 0 && live) {  // same rows, resume point updated
+      write_rec(a, sm.rec, sm.task);
+      sm.push[np++] = sm.rec;
     } else if (k >= 2) {
-      free_push(a, t.block);
+      blk_free(a, sm, t.block);
     }
-    // lexicographically first rows on top: newest (last) block first
+    // lexicographically first rows on top: the last block is pushed first
     if (sm.out_block != kNil) {
-      if (sm.out_fill) push_task(a, sm.out_block, Task{t.q, kout, sm.out_block, 0, sm.out_fill, 0, 0, kNil});
-      else free_push(a, sm.out_block);
+      if (fill && live) {
+        write_rec(a, sm.out_block, Task{t.q, kout, sm.out_block, 0, fill, 0, 0, 0});
+        sm.push[np++] = sm.out_block;
+      } else {
+        blk_free(a, sm, sm.out_block);
+      }
     }
-    for (int i = static_cast<int>(sm.nheld) - 1; i >= 0; --i)
-      push_task(a, sm.held[i], Task{t.q, kout, sm.held[i], 0, sm.held_fill[i], 0, 0, kNil});
-    if (count_level && tot_contrib) {
-      const unsigned long long c = atomicAdd(&a.st[t.q].count, tot_contrib) + tot_contrib;
-      if (c > Q.cap) atomicExch(&a.st[t.q].done, 1u);
+    for (int i = static_cast<int>(sm.nheld) - 1; i >= 0; --i) {
+      if (live) {
+        write_rec(a, sm.held[i], Task{t.q, kout, sm.held[i], 0, sm.held_fill[i], 0, 0, 0});
+        sm.push[np++] = sm.held[i];
+      } else {
+        blk_free(a, sm, sm.held[i]);
+      }
     }
-    atomicAdd(&a.ctl->items, tot_items);
-    atomicAdd(&a.ctl->rows_in, rows_seen);
-    atomicAdd(&a.ctl->rows_out, tot_written);
-    atomicAdd(&a.ctl->tasks, 1ull);
-    if (sm.flag == 2) sm.cur_q = (sm.cur_q + 1) % a.nqueries;  // let this query's pool users finish
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    deque_push(a, wg, sm.push, np);
+    sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
+    if (sm.acc_q != t.q) {
+      flush_acc(a, sm);
+      sm.acc_q = t.q;
+    }
+    sm.acc_count += count_level ? tot_contrib : 0;
+    sm.acc_items += tot_items;
+    sm.acc_rows += rows_written;
+    if (sm.acc_count >= (Q.cap >> 8) + 1 || sm.acc_items >= (1ull << 22)) flush_acc(a, sm);
+    n_items += tot_items;
+    n_rows_out += rows_written;
   }
   __syncthreads();
 }
 
+__device__ inline uint32_t xorshift(uint32_t &s) {
+  s ^= s << 13;
+  s ^= s >> 17;
+  s ^= s << 5;
+  return s;
+}
+
 __global__ __launch_bounds__(kThreads) void batch_join_kernel(BArgs a) {
-  extern __shared__ uint32_t s_rows[];  // kRowBatch x max_k
+  extern __shared__ uint32_t s_rows[];  // kRowBatch x widest row
   __shared__ Smem sm;
   const int tid = threadIdx.x;
+  const uint32_t wg = blockIdx.x, nwgs = gridDim.x;
   const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
-  if (tid == 0) sm.cur_q = blockIdx.x % a.nqueries;
+  const uint64_t nroot = static_cast<uint64_t>(a.nqueries) * a.tasks_per_query;
+  uint64_t busy = 0;
+  unsigned long long n_items = 0, n_rows_in = 0, n_rows_out = 0, n_tasks = 0, n_steals = 0;
+  uint32_t rng = 0x9E3779B9u ^ (wg * 0x85EBCA6Bu + 1u);
+  bool idle = false;  // counted in ctl->idle
+  uint32_t idle_rounds = 0;
+  if (tid == 0) {
+    sm.staged_q = kNil;
+    sm.staged_k = kNil;
+    sm.ncache = 0;
+    sm.acc_q = kNil;
+    sm.acc_count = sm.acc_items = sm.acc_rows = 0;
+    sm.sync_ticks = 0;
+  }
   for (;;) {
     if (tid == 0) {
       if (__builtin_amdgcn_s_memrealtime() - t_start > a.deadline_ticks) atomicExch(&a.ctl->abort, 1u);
       sm.state = 1;
-      for (uint32_t i = 0; i < a.nqueries; ++i) {
-        const uint32_t q = (sm.cur_q + i) % a.nqueries;
-        uint32_t rec;
-        if (stack_pop(a, q, rec)) {
-          const Task &r = a.recs[rec];
-          sm.task.q = r.q;
-          sm.task.k = r.k;
-          sm.task.block = r.block;
-          sm.task.r0 = r.r0;
-          sm.task.r1 = r.r1;
-          sm.task.skip_rows = r.skip_rows;
-          sm.task.skip_items = r.skip_items;
-          sm.rec = rec;
-          sm.cur_q = q;
-          sm.state = 0;
+      const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+      // own deque, then a new root task, then the oldest task of another workgroup
+      uint32_t rec = idle ? kNil : deque_pop(a, wg);  // an idle workgroup's deque is empty
+      bool root = false;
+      if (rec == kNil && ld_rlx(&a.ctl->root_next) < nroot) {
+        if (idle) {
+          atomicAdd(&a.ctl->idle, ~0ull);
+          idle = false;
+        }
+        for (;;) {
+          const unsigned long long i = atomicAdd(&a.ctl->root_next, 1ull);
+          if (i >= nroot) break;
+          // root tasks interleave the queries; each query's ranges ascend
+          const uint32_t q = static_cast<uint32_t>(i % a.nqueries), c = static_cast<uint32_t>(i / a.nqueries);
+          if (ld_done(a, q) != kRunning) continue;
+          rec = static_cast<uint32_t>(a.num_blocks + static_cast<uint64_t>(q) * a.tasks_per_query + c);
+          sm.task = Task{q, 1, 0, c * a.root_words, min(a.words, (c + 1) * a.root_words), 0, 0, 0};
+          root = true;
           break;
         }
       }
-      if (sm.state == 1 &&
-          (ld_rlx(&a.ctl->outstanding) == 0 || __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-        sm.state = 2;
-      if (sm.state == 0)
-        sm.stop = __hip_atomic_load(&a.st[sm.task.q].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                  __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      if (rec == kNil) {
+        for (uint32_t i = 0; i < 8 && rec == kNil; ++i) {
+          const uint32_t v = xorshift(rng) % nwgs;
+          if (v != wg) rec = deque_steal(a, v, idle);
+        }
+        if (rec != kNil) {
+          ++n_steals;
+          idle = false;
+        }
+      }
+      if (rec != kNil) {
+        if (!root) sm.task = a.recs[rec];
+        sm.rec = rec;
+        sm.state = 0;
+        sm.stop = ld_done(a, sm.task.q) != kRunning || aborted(a);
+      } else {
+        if (!idle) {
+          atomicAdd(&a.ctl->idle, 1ull);
+          idle = true;
+        }
+        if (aborted(a) || (ld_rlx(&a.ctl->root_next) >= nroot && ld_rlx(&a.ctl->idle) == nwgs)) sm.state = 2;
+      }
+      sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
     }
     __syncthreads();
     const int state = sm.state;
     if (state == 2) break;
-    if (state == 1) {
-      __builtin_amdgcn_s_sleep(8);
+    if (state == 1) {  // back off while the other workgroups produce work
+      if (tid == 0) flush_acc(a, sm);
+      idle_rounds = min(idle_rounds + 1, 16u);
+      for (uint32_t i = 0; i < idle_rounds; ++i) __builtin_amdgcn_s_sleep(16);
       continue;
     }
-    __threadfence();  // acquire the task's rows written by another workgroup
-    const uint32_t q = sm.task.q;
-    {  // stage the query header and the level descriptors
+    idle_rounds = 0;
+    const uint64_t t_task = __builtin_amdgcn_s_memrealtime();
+    // a stolen task's rows were published by the victim's unlock and acquired
+    // by the lock of the steal (thread 0); the barrier orders the other reads
+    const uint32_t q = sm.task.q, k = sm.task.k;
+    if (sm.staged_q != q) {  // stage the query header (uniform branch)
       const uint32_t *src = reinterpret_cast<const uint32_t *>(&a.qs[q]);
       uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.q);
       for (uint32_t i = tid; i < sizeof(BQuery) / 4; i += kThreads) dst[i] = src[i];
-    }
-    __syncthreads();
-    {
-      const uint32_t k = sm.task.k;
-      const uint32_t *src = reinterpret_cast<const uint32_t *>(&sm.q.lv[k]);
-      uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.lv);
-      for (uint32_t i = tid; i < sizeof(LevelJoin) / 4; i += kThreads) dst[i] = src[i];
+      __syncthreads();
       if (sm.q.fused) {
         const uint32_t *s2 = reinterpret_cast<const uint32_t *>(&sm.q.lv[sm.q.nq - 1]);
         uint32_t *d2 = reinterpret_cast<uint32_t *>(&sm.lv_last);
         for (uint32_t i = tid; i < sizeof(LevelJoin) / 4; i += kThreads) d2[i] = s2[i];
       }
     }
+    if (sm.staged_q != q || sm.staged_k != k) {  // and the level descriptor
+      const uint32_t *src = reinterpret_cast<const uint32_t *>(&sm.q.lv[k]);
+      uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.lv);
+      for (uint32_t i = tid; i < sizeof(LevelJoin) / 4; i += kThreads) dst[i] = src[i];
+    }
     __syncthreads();
+    if (tid == 0) {
+      sm.staged_q = q;
+      sm.staged_k = k;
+    }
     if (!sm.stop) {
-      process_task(a, sm, s_rows);
-    } else if (tid == 0 && sm.task.k >= 2) {
-      free_push(a, sm.task.block);
+      process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out);
+      if (sm.flag == 2)  // pool empty: give other workgroups time to free blocks
+        for (int i = 0; i < 16; ++i) __builtin_amdgcn_s_sleep(32);
+    } else if (tid == 0 && k >= 2) {
+      blk_free(a, sm, sm.task.block);
     }
     __syncthreads();
-    if (tid == 0) {  // this task is finished
-      atomicSub(&a.st[q].pending, 1u);
-      atomicAdd(&a.ctl->outstanding, ~0ull);
-    }
+    ++n_tasks;
+    busy += __builtin_amdgcn_s_memrealtime() - t_task;
+  }
+  if (tid == 0) {
+    flush_acc(a, sm);
+    const uint64_t idle_t = __builtin_amdgcn_s_memrealtime() - t_start - busy;
+    atomicAdd(&a.ctl->busy_ticks, static_cast<unsigned long long>(busy));
+    atomicAdd(&a.ctl->idle_ticks, static_cast<unsigned long long>(idle_t));
+    atomicAdd(&a.ctl->sync_ticks, sm.sync_ticks);
+    atomicAdd(&a.ctl->items, n_items);
+    atomicAdd(&a.ctl->rows_in, n_rows_in);
+    atomicAdd(&a.ctl->rows_out, n_rows_out);
+    atomicAdd(&a.ctl->tasks, n_tasks);
+    atomicAdd(&a.ctl->steals, n_steals);
   }
 }
 
 uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(65536, (n + 255) / 256 + 1)); }
 
 struct TuneParams {
-  uint32_t wgs_per_cu, root_words, block_kb;
+  uint32_t wgs_per_cu, root_words, block_kb, ilp, max_out;
 };
 
 }  // namespace
@@ -712,7 +954,12 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_bq, bq.data(), sizeof(BQuery) * nqs, hipMemcpyHostToDevice, s));
 
   // 4. queues and pool
-  TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, 128u};
+  TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 2u, 16u, 128u, kIlp, 4u};
+  uint64_t budget = 2000000000ull;
+  if (const char *e = std::getenv("CSM_BATCH_BUDGET")) budget = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
+  if (const char *e = std::getenv("CSM_BATCH_MAX_OUT")) tp.max_out = std::max(1, std::atoi(e));
+  if (const char *e = std::getenv("CSM_BATCH_ILP")) tp.ilp = std::min<uint32_t>(kIlp, std::max(1, std::atoi(e)));
+  if (const char *e = std::getenv("CSM_BATCH_BLOCK_KB")) tp.block_kb = std::max(32, std::atoi(e));
   if (o.chunk_rows) tp.root_words = std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, o.chunk_rows / 64));
   const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;
   if (block_words < static_cast<uint64_t>(kThreads) * CSM_MAX_QV) return set_error(ctx, CSM_E_ARG, "block too small");
@@ -724,24 +971,30 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   const uint32_t tasks_per_query = ceil_div(words, tp.root_words);
   const uint64_t nroot = static_cast<uint64_t>(nqs) * tasks_per_query;
   if (num_blocks + nroot >= kNil) return set_error(ctx, CSM_E_LIMIT, "too many pool blocks and root tasks");
+  const uint32_t grid = ctx->num_cus * tp.wgs_per_cu;
   uint32_t *pool = static_cast<uint32_t *>(get(4 * block_words * num_blocks));
   Task *recs = static_cast<Task *>(get(sizeof(Task) * (num_blocks + nroot)));
-  QStack *stacks = static_cast<QStack *>(get(sizeof(QStack) * nqs));
+  Deque *dq = static_cast<Deque *>(get(sizeof(Deque) * grid));
+  uint32_t *dq_buf = static_cast<uint32_t *>(get(4ull * kDequeCap * grid));
   FreeSlot *free_ring = static_cast<FreeSlot *>(get(sizeof(FreeSlot) * num_blocks));
   unsigned long long *ctr = static_cast<unsigned long long *>(get(8ull * 2));
   BCtl *d_ctl = static_cast<BCtl *>(get(sizeof(BCtl)));
-  if (!pool || !recs || !stacks || !free_ring || !ctr || !d_ctl)
+  if (!pool || !recs || !dq || !dq_buf || !free_ring || !ctr || !d_ctl)
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (batch pool)");
   BArgs a{};
-  a.off = g->off;
-  a.adj = g->adj;
   a.probe = AdjProbe{g->off, g->adj, g->hub_id, g->hub_bits, words};
   a.qs = d_bq;
   a.st = d_st;
   a.ctl = d_ctl;
   a.recs = recs;
-  a.stacks = stacks;
+  a.dq = dq;
+  a.dq_buf = dq_buf;
   a.nqueries = nqs;
+  a.tasks_per_query = tasks_per_query;
+  a.words = words;
+  a.budget = budget;
+  // workgroup caches hold at most a quarter of the pool
+  a.cache_cap = static_cast<uint32_t>(std::min<uint64_t>(kCache, num_blocks / (4ull * grid)));
   a.free_head = ctr;
   a.free_tail = ctr + 1;
   a.free_ring = free_ring;
@@ -749,12 +1002,14 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   a.num_blocks = num_blocks;
   a.block_words = block_words;
   a.root_words = tp.root_words;
-  a.max_k = max_nq;
-  a.deadline_ticks = 100ull * 1000 * 1000 * 120;  // 120 s at the 100 MHz real-time clock
+  a.ilp = tp.ilp;
+  a.max_out = std::min(tp.max_out, kMaxOut);
+  double limit_s = 120;
+  if (const char *e = std::getenv("CSM_BATCH_TIME_LIMIT")) limit_s = std::max(1.0, std::atof(e));
+  a.deadline_ticks = static_cast<uint64_t>(limit_s * 1e8);  // 100 MHz real-time clock
   const size_t lds_rows = 4ull * kRowBatch * std::max<uint32_t>(1, max_nq - 1);
-  const uint32_t grid = ctx->num_cus * tp.wgs_per_cu;
   auto launch_join = [&]() {
-    batch_init_kernel<<<grid_for(std::max<uint64_t>(num_blocks, nroot)), 256, 0, s>>>(a, tasks_per_query, words);
+    batch_init_kernel<<<grid_for(std::max<uint64_t>(num_blocks, grid)), 256, 0, s>>>(a, grid);
     batch_join_kernel<<<grid, kThreads, lds_rows, s>>>(a);
   };
   if (o.use_graph) {
@@ -785,6 +1040,15 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   // lookups, list entries, rows written
   const uint64_t join_bytes = hctl.rows_in * (4ull * 8 + 32) + hctl.items * 4 + hctl.rows_out * 4ull * 9;
   kstat[CSM_K_BATCH].bytes = join_bytes;
+  const char *tr = std::getenv("CSM_TRACE");
+  const bool trace = tr && *tr == '1';
+  if (trace)
+    std::fprintf(stderr,
+                 "csm batch: queries %u grid %u blocks %llu tasks %llu items %llu rows_in %llu rows_out %llu "
+                 "steals %llu abort %u overflow %u fails %llu; per workgroup: busy %.1f ms idle %.1f ms sync %.1f ms\n",
+                 nqs, grid, static_cast<unsigned long long>(num_blocks), hctl.tasks, hctl.items, hctl.rows_in,
+                 hctl.rows_out, hctl.steals, hctl.abort, hctl.overflow, hctl.fails, hctl.busy_ticks / 1e5 / grid,
+                 hctl.idle_ticks / 1e5 / grid, hctl.sync_ticks / 1e5 / grid);
   const double ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (uint32_t i = 0; i < nqs; ++i) {
     csm_result &r = results[i];
@@ -796,15 +1060,19 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
     r.ms_filter = ms_filter;
     r.ms_total = ms_total;
     r.ms_join = ms_total - ms_filter;
+    r.items = hst[i].items;
+    r.partial_rows = hst[i].rows;
     if (i == 0) {
       for (int c = 0; c < CSM_K_NUM; ++c) r.kernels[c] = kstat[c];
-      r.items = hctl.items;
-      r.partial_rows = hctl.rows_out;
+      r.kernels[CSM_K_BATCH].launches = 1;
     }
   }
-  if (hctl.abort) {  // pool exhausted without progress: finish unfinished queries one by one
-    for (uint32_t i = 0; i < nqs; ++i) {
-      if (!bq[i].active || hst[i].done || hst[i].pending == 0) continue;
+  // queries over their budget, and every unfinished query after an abort,
+  // are finished by the plain path (one at a time, exact)
+  uint32_t fallbacks = 0;
+  for (uint32_t i = 0; i < nqs; ++i) {
+    if (!bq[i].active || hst[i].done == kCapped) continue;
+    if (hst[i].done == kAbandoned || hctl.abort) {
       csm_match_opts mo{};
       mo.cap = o.cap;
       mo.refine = o.refine;
@@ -814,8 +1082,15 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
       if (st != CSM_OK) return st;
       results[i].count = r.count;
       results[i].capped = r.capped;
+      ++fallbacks;
     }
   }
+  const double ms_all = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (uint32_t i = 0; i < nqs; ++i) {
+    results[i].ms_total = ms_all;
+    results[i].ms_join = ms_all - ms_filter;
+  }
+  if (trace) std::fprintf(stderr, "csm batch: %u queries finished by the plain path, %.1f ms in all\n", fallbacks, ms_all);
   return CSM_OK;
 }
 
