@@ -29,6 +29,7 @@ E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING = 2, 3, 4, 7, 8, 9
 
 K_FILTER, K_REFINE, K_EXPAND, K_MATERIALIZE, K_COUNT, K_AUX, K_BATCH = range(7)
 K_NUM = 7
+PATH_PLAIN, PATH_HOST, PATH_BATCH, PATH_BATCH_BUDGET, PATH_BATCH_ABORT = range(5)
 KERNEL_NAMES = ["filter", "refine", "expand", "materialize", "count", "aux", "batch"]
 
 
@@ -105,7 +106,8 @@ class Result(C.Structure):
     _fields_ = [("count", C.c_uint64), ("capped", C.c_uint32), ("status", C.c_uint32),
                 ("listed", C.c_uint64), ("partial_rows", C.c_uint64), ("items", C.c_uint64),
                 ("ms_filter", C.c_double), ("ms_join", C.c_double), ("ms_total", C.c_double),
-                ("ms_latency", C.c_double), ("kernels", KernelStat * K_NUM)]
+                ("ms_latency", C.c_double), ("path", C.c_uint32), ("reserved", C.c_uint32),
+                ("kernels", KernelStat * K_NUM)]
 
     def kernel(self, cls):
         k = self.kernels[cls]
@@ -114,7 +116,9 @@ class Result(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("use_graph", C.c_uint32),
-                ("chunk_rows", C.c_uint32), ("wgs_per_cu", C.c_uint32), ("pool_bytes", C.c_uint64)]
+                ("chunk_rows", C.c_uint32), ("wgs_per_cu", C.c_uint32), ("pool_bytes", C.c_uint64),
+                ("block_kb", C.c_uint32), ("max_out", C.c_uint32), ("ilp", C.c_uint32),
+                ("time_limit_ms", C.c_uint32), ("item_budget", C.c_uint64)]
 
 
 class ServeOpts(C.Structure):
@@ -435,9 +439,12 @@ class Query:
         return r, rows
 
 
-def batch_match(graph, queries, cap=0, refine=True, use_graph=True, chunk_rows=0, wgs_per_cu=0, pool_bytes=0):
+def batch_match(graph, queries, cap=0, refine=False, use_graph=True, chunk_rows=0, wgs_per_cu=0, pool_bytes=0,
+                block_kb=0, max_out=0, ilp=0, time_limit_ms=0, item_budget=0):
+    """All queries in one persistent kernel (csm_batch_match); counts only."""
     arr = (_vp * len(queries))(*[q.h for q in queries])
-    o = BatchOpts(cap, 1 if refine else 0, 1 if use_graph else 0, chunk_rows, wgs_per_cu, pool_bytes)
+    o = BatchOpts(cap, 1 if refine else 0, 1 if use_graph else 0, chunk_rows, wgs_per_cu, pool_bytes,
+                  block_kb, max_out, ilp, time_limit_ms, item_budget)
     res = (Result * len(queries))()
     graph.ctx.check(lib().csm_batch_match(graph.ctx.h, graph.h, arr, len(queries), C.byref(o), res),
                     "csm_batch_match")

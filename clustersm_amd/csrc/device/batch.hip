@@ -79,14 +79,18 @@ struct Task {
 constexpr uint32_t kNil = 0xFFFFFFFFu;
 constexpr uint32_t kDequeCap = 1024;  // > levels x (held blocks + 2): a deque never fills
 
-// One task deque per workgroup.  The owner pushes and pops at `bottom`
-// (depth-first, lexicographically first rows on top); other workgroups
-// steal the oldest task at `top`.  A short lock guards both ends.
+// One task deque per workgroup, split in two: entries [top, pub) are
+// published and may be stolen (oldest first, at `top`); entries [pub,
+// bottom) are private to the owner, which pushes and pops them at `bottom`
+// (depth-first, lexicographically first rows on top) with no atomics and
+// no fences.  The owner publishes, with one release, only while some
+// workgroup is idle.  A short lock orders thieves with the owner's pops
+// from the published part.  `bottom` lives in the owner's LDS.
 struct Deque {
   unsigned int lock;
   unsigned int pad;
   unsigned long long top;
-  unsigned long long bottom;
+  unsigned long long pub;
   unsigned long long pad2;
 };
 
@@ -98,6 +102,7 @@ struct FreeSlot {
 struct BCtl {
   unsigned long long root_next;    // next root task to hand out
   unsigned long long idle;         // workgroups without a task
+  unsigned long long started;      // workgroups that have begun (all of them unless the GPU is shared)
   unsigned long long items;        // (row, list entry) pairs examined
   unsigned long long rows_in;      // rows read
   unsigned long long rows_out;     // rows written
@@ -107,6 +112,7 @@ struct BCtl {
   unsigned long long busy_ticks;   // real-time ticks inside tasks, summed over workgroups
   unsigned long long idle_ticks;   // real-time ticks without a task
   unsigned long long sync_ticks;   // real-time ticks in deque and free-ring operations
+  unsigned long long phase_ticks[4];  // rows staged + drivers, count tiles, extension tiles, task ends
   unsigned long long alloc_fail;   // consecutive failed allocations from the free ring
   unsigned int abort;
   unsigned int overflow;           // a deque was full (never expected)
@@ -165,50 +171,19 @@ __device__ inline void dunlock(unsigned int *l) {
   __hip_atomic_store(l, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Owner: push n records in order (the last one ends on top).  Record
-// fields and their rows are written before; the unlock publishes them.
-__device__ void deque_push(const BArgs &a, uint32_t wg, const uint32_t *recs, uint32_t n) {
-  if (n == 0) return;
-  Deque *d = &a.dq[wg];
-  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
-  dlock(&d->lock);
-  const unsigned long long t = ld_rlx(&d->top), b = ld_rlx(&d->bottom);
-  if (b - t + n > kDequeCap) {
-    atomicExch(&a.ctl->overflow, 1u);
-    atomicExch(&a.ctl->abort, 1u);
-  } else {
-    for (uint32_t i = 0; i < n; ++i) buf[(b + i) % kDequeCap] = recs[i];
-    st_rlx(&d->bottom, b + n);
-  }
-  dunlock(&d->lock);
-}
-
-// Owner: newest record, or kNil.
-__device__ uint32_t deque_pop(const BArgs &a, uint32_t wg) {
-  Deque *d = &a.dq[wg];
-  if (ld_rlx(&d->bottom) == ld_rlx(&d->top)) return kNil;
-  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
-  dlock(&d->lock);
-  const unsigned long long t = ld_rlx(&d->top), b = ld_rlx(&d->bottom);
-  uint32_t r = kNil;
-  if (b > t) {
-    r = buf[(b - 1) % kDequeCap];
-    st_rlx(&d->bottom, b - 1);
-  }
-  dunlock(&d->lock);
-  return r;
-}
-
 // Thief: oldest record of workgroup v, or kNil.  A thief counted idle
 // leaves the idle count while it holds v's lock: v can only declare itself
 // idle after the unlock, so the count never reaches every workgroup while a
 // task is in hand.
 __device__ uint32_t deque_steal(const BArgs &a, uint32_t v, bool counted_idle) {
   Deque *d = &a.dq[v];
-  if (ld_rlx(&d->bottom) <= ld_rlx(&d->top)) return kNil;
+  if (ld_rlx(&d->pub) <= ld_rlx(&d->top)) return kNil;
   uint32_t *buf = a.dq_buf + static_cast<uint64_t>(v) * kDequeCap;
   dlock(&d->lock);
-  const unsigned long long t = ld_rlx(&d->top), b = ld_rlx(&d->bottom);
+  // acquire: the entries below pub, their records and rows were written
+  // before the owner's release of pub
+  const unsigned long long b = __hip_atomic_load(&d->pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long t = ld_rlx(&d->top);
   uint32_t r = kNil;
   if (b > t) {
     r = buf[t % kDequeCap];
@@ -260,7 +235,7 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
   for (uint64_t i = tid; i < nwgs; i += stride) {
     a.dq[i].lock = 0;
     a.dq[i].top = 0;
-    a.dq[i].bottom = 0;
+    a.dq[i].pub = 0;
   }
   for (uint64_t i = tid; i < a.nqueries; i += stride) {
     a.st[i].count = 0;
@@ -271,8 +246,9 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     *a.free_head = 0;
     *a.free_tail = a.num_blocks;
     BCtl &c = *a.ctl;
-    c.root_next = c.idle = c.items = c.rows_in = c.rows_out = c.tasks = c.steals = c.fails = 0;
+    c.root_next = c.idle = c.started = c.items = c.rows_in = c.rows_out = c.tasks = c.steals = c.fails = 0;
     c.busy_ticks = c.idle_ticks = c.sync_ticks = 0;
+    c.phase_ticks[0] = c.phase_ticks[1] = c.phase_ticks[2] = c.phase_ticks[3] = 0;
     c.alloc_fail = 0;
     c.abort = c.overflow = 0;
   }
@@ -282,6 +258,8 @@ constexpr uint32_t kIlp = 4;     // items per thread per tile, checked in lockst
 constexpr uint32_t kMaxA = 4;    // anchors whose candidate lists are kept per row
 constexpr uint32_t kMaxOut = 8;  // most full output blocks a task holds before it yields
 constexpr uint32_t kCache = 16;  // free blocks a workgroup keeps for itself
+
+static_assert(kRowBatch <= 256 && kRowBatch <= kThreads, "row index in a batch is a byte");
 
 struct Smem {
   Task task;
@@ -293,7 +271,14 @@ struct Smem {
   uint64_t abeg[kRowBatch][kMaxA];  // candidate list of each kept anchor: begin
   uint32_t alen[kRowBatch][kMaxA];  // and length
   uint64_t dbeg[kRowBatch];         // driver list start (after a resume skip)
+  // fused count, when the last position's parent is an earlier column: per
+  // row, |N(x) ∩ C(u_last)| minus the row's own members of that list, and the list
+  uint32_t fbase[kRowBatch];
+  uint32_t flen[kRowBatch];
+  uint64_t fbeg[kRowBatch];
   uint8_t drv[kRowBatch];
+  uint8_t out_ri[kThreads * kIlp];  // a tile's survivors in output order: row in the batch
+  uint32_t out_w[kThreads * kIlp];  // and the new id
   unsigned long long red[kWaves];
   uint32_t tile_n[3];               // rotating survivor counters, one per tile in flight
   uint32_t out_block;
@@ -305,10 +290,13 @@ struct Smem {
   uint32_t push[kMaxOut + 2];       // records pushed when the task ends
   uint32_t cache[kCache];           // free blocks owned by this workgroup
   uint32_t ncache;
+  unsigned long long dq_bottom;     // owner's end of its deque
+  unsigned long long dq_pub;        // published bound (only the owner changes it)
   // counts and statistics not yet added to the query's totals
   uint32_t acc_q;
   unsigned long long acc_count, acc_items, acc_rows;
   unsigned long long sync_ticks;
+  unsigned long long phase[4];
   int flag;                         // 0 go on, 1 output bound reached, 2 pool empty: yield with continuation
   int state;                        // loop control: 0 work, 1 idle, 2 exit
   int stop;                         // broadcast: the task's query is finished or the kernel aborts
@@ -338,6 +326,52 @@ __device__ void blk_free(const BArgs &a, Smem &sm, uint32_t id) {
   const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
   free_push(a, id);
   sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
+}
+
+// Owner (thread 0): entries [pub, upto) become stealable.  Everything this
+// workgroup wrote before (entries, records, rows, after the barrier that
+// ended their task) is released with them.
+__device__ void deque_publish(const BArgs &a, Smem &sm, uint32_t wg, unsigned long long upto) {
+  __hip_atomic_store(&a.dq[wg].pub, upto, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  sm.dq_pub = upto;
+}
+
+// Owner: push n records in order (the last one ends on top), privately;
+// published at once when a workgroup is idle.
+__device__ void deque_push(const BArgs &a, Smem &sm, uint32_t wg, const uint32_t *recs, uint32_t n) {
+  if (n == 0) return;
+  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
+  const unsigned long long b = sm.dq_bottom, t = ld_rlx(&a.dq[wg].top);
+  if (b - t + n > kDequeCap) {
+    atomicExch(&a.ctl->overflow, 1u);
+    atomicExch(&a.ctl->abort, 1u);
+    return;
+  }
+  for (uint32_t i = 0; i < n; ++i) buf[(b + i) % kDequeCap] = recs[i];
+  sm.dq_bottom = b + n;
+  if (ld_rlx(&a.ctl->idle) > 0) deque_publish(a, sm, wg, sm.dq_bottom);
+}
+
+// Owner: newest record, or kNil.  Private entries need no synchronisation;
+// while others are idle, all but the one taken are published first.
+__device__ uint32_t deque_pop(const BArgs &a, Smem &sm, uint32_t wg) {
+  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
+  if (sm.dq_bottom > sm.dq_pub) {
+    if (sm.dq_bottom - sm.dq_pub > 1 && ld_rlx(&a.ctl->idle) > 0) deque_publish(a, sm, wg, sm.dq_bottom - 1);
+    return buf[(--sm.dq_bottom) % kDequeCap];
+  }
+  Deque *d = &a.dq[wg];
+  if (sm.dq_pub <= ld_rlx(&d->top)) return kNil;
+  dlock(&d->lock);
+  const unsigned long long t = ld_rlx(&d->top), p = sm.dq_pub;
+  uint32_t r = kNil;
+  if (p > t) {
+    r = buf[(p - 1) % kDequeCap];
+    st_rlx(&d->pub, p - 1);
+    sm.dq_pub = sm.dq_bottom = p - 1;
+  }
+  dunlock(&d->lock);
+  return r;
 }
 
 // Thread 0: adds the workgroup's pending count and statistics to query
@@ -387,8 +421,6 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t *lds, uint64_t &
   return off + x - v;
 }
 
-__device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
-
 __device__ inline uint32_t row_of(const Smem &sm, uint32_t nb, uint64_t it) {
   uint32_t lo = 0, hi = nb;
   while (hi - lo > 1) {
@@ -396,6 +428,41 @@ __device__ inline uint32_t row_of(const Smem &sm, uint32_t nb, uint64_t it) {
     if (sm.pre[mid] <= it) lo = mid; else hi = mid;
   }
   return lo;
+}
+
+// Membership of v[j] in the sorted lists cn[lo[j] .. lo[j] + n[j]), all
+// searches advancing together so their loads overlap.  n[j] = 0: not found.
+template <uint32_t N>
+__device__ inline void lockstep_find(const uint32_t *cn, uint64_t (&lo)[N], uint32_t (&n)[N], const uint32_t (&v)[N],
+                                     bool (&found)[N]) {
+  bool has[N];
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) has[j] = n[j] != 0;
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) any |= n[j] > 1;
+    if (!any) break;
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+      if (n[j] > 1) {
+        const uint32_t half = n[j] >> 1;
+        if (cn[lo[j] + half] <= v[j]) lo[j] += half;
+        n[j] -= half;
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) found[j] = has[j] && cn[lo[j]] == v[j];
+}
+
+__device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, uint32_t v) {
+  uint64_t lo[1] = {beg};
+  uint32_t n[1] = {len};
+  const uint32_t vv[1] = {v};
+  bool f[1];
+  lockstep_find<1>(cn, lo, n, vv, f);
+  return f[0];
 }
 
 __device__ inline void write_rec(const BArgs &a, uint32_t rec, const Task &t) {
@@ -420,7 +487,15 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   const uint32_t kout = k + 1;
   const uint32_t out_cap = static_cast<uint32_t>(a.block_words / kout);
   const uint32_t ilp = count_level ? kIlp : min(a.ilp, out_cap / kThreads);
-  const uint32_t nanchor = sm.lv.nanchor;
+  // i / kout == __umulhi(i, kout_magic) for every word index of a tile (< 2^17)
+  const uint32_t kout_magic = static_cast<uint32_t>((0x100000000ull + kout - 1) / kout);
+  const uint32_t nanchor = sm.lv.nanchor, nkept = min(nanchor, kMaxA);
+  const bool fused_rows = count_level && Q.fused && sm.lv_last.parent != k;
+  bool last_inj_k = false;  // the last position's label equals this one's (lv_last is staged only when fused)
+  if (Q.fused)
+    for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) last_inj_k |= sm.lv_last.inj[x] == k;
+  // count-level rows whose count needs no per-item test
+  const bool row_counted = count_level && nanchor == 1 && (!Q.fused || (fused_rows && !last_inj_k));
   // rows of the task
   uint32_t nrows;
   const uint32_t *grows = nullptr;
@@ -459,6 +534,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     sm.tile_n[0] = sm.tile_n[1] = sm.tile_n[2] = 0;
   }
   __syncthreads();
+  uint64_t tp0 = __builtin_amdgcn_s_memrealtime();
   for (; row0 < nrows; row0 += kRowBatch) {
     if (tid == 0) sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
     __syncthreads();
@@ -467,19 +543,31 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     for (uint32_t i = tid; i < nb * k; i += kThreads)
       s_rows[i] = k == 1 ? sm.roots[row0 + i] : grows[static_cast<uint64_t>(row0) * k + i];
     __syncthreads();
-    // each row: candidate lists of its anchors; the shortest drives
+    // candidate lists of the kept anchors, one (row, anchor) pair per thread
+    for (uint32_t p = tid; p < nb * nkept; p += kThreads) {
+      const uint32_t r = p / nkept, x = p - r * nkept;
+      uint64_t b0, e0;
+      Q.ci.list(sm.lv, x, s_rows[r * k + sm.lv.anchor[x]], b0, e0);
+      sm.abeg[r][x] = b0;
+      sm.alen[r][x] = static_cast<uint32_t>(e0 - b0);
+    }
+    __syncthreads();
+    // each row: the shortest list drives
     uint64_t len = 0;
     if (static_cast<uint32_t>(tid) < nb) {
       const uint32_t *row = s_rows + tid * k;
       uint64_t best = ~0ull, bbeg = 0;
       uint32_t bi = 0;
-      for (uint32_t x = 0; x < nanchor; ++x) {
+      for (uint32_t x = 0; x < nkept; ++x) {
+        if (sm.alen[tid][x] < best) {
+          best = sm.alen[tid][x];
+          bbeg = sm.abeg[tid][x];
+          bi = x;
+        }
+      }
+      for (uint32_t x = kMaxA; x < nanchor; ++x) {
         uint64_t b0, e0;
         Q.ci.list(sm.lv, x, row[sm.lv.anchor[x]], b0, e0);
-        if (x < kMaxA) {
-          sm.abeg[tid][x] = b0;
-          sm.alen[tid][x] = static_cast<uint32_t>(e0 - b0);
-        }
         if (e0 - b0 < best) {
           best = e0 - b0;
           bbeg = b0;
@@ -493,6 +581,29 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       sm.dbeg[tid] = bbeg;
       sm.drv[tid] = static_cast<uint8_t>(bi);
       len = best;
+      if (fused_rows) {  // the row's part of the last position's count
+        const uint32_t xv = row[sm.lv_last.parent];
+        uint64_t lb, le;
+        Q.ci.list(sm.lv_last, 0, xv, lb, le);
+        uint32_t c = static_cast<uint32_t>(le - lb);
+        for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) {
+          const uint32_t col = sm.lv_last.inj[x];
+          if (col == k) continue;
+          const uint32_t v = row[col];
+          if (v != xv && in_list(Q.ci.cn_adj, lb, static_cast<uint32_t>(le - lb), v)) --c;
+        }
+        sm.fbase[tid] = c;
+        sm.fbeg[tid] = lb;
+        sm.flen[tid] = static_cast<uint32_t>(le - lb);
+      }
+      if (row_counted) {  // one anchor: the row's survivors are its list minus the row's own ids
+        uint64_t surv = len;
+        for (uint32_t x = 0; x < sm.lv.ninj; ++x)
+          if (in_list(Q.ci.cn_adj, bbeg, static_cast<uint32_t>(len), row[sm.lv.inj[x]])) --surv;
+        contrib += surv * (fused_rows ? sm.fbase[tid] : 1u);
+        items_seen += len;
+        len = 0;
+      }
     }
     const uint32_t first_skip = skip_items;  // items of row row0 already done before this batch
     skip_items = 0;
@@ -502,6 +613,11 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     if (tid == 0) sm.pre[nb] = total;
     __syncthreads();
     n_rows_in += nb;
+    if (tid == 0) {
+      const uint64_t tn = __builtin_amdgcn_s_memrealtime();
+      sm.phase[0] += tn - tp0;
+      tp0 = tn;
+    }
     const uint64_t step = static_cast<uint64_t>(kThreads) * ilp;
     for (uint64_t base = 0; base < total; base += step) {
       uint32_t w[kIlp], ri[kIlp];
@@ -532,32 +648,17 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         if (x < kMaxA) {
           uint64_t lo[kIlp];
           uint32_t n[kIlp];
+          bool f[kIlp];
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j) {
             const bool act = ok[j] && sm.drv[ri[j]] != x;
             lo[j] = sm.abeg[ri[j]][x];
             n[j] = act ? sm.alen[ri[j]][x] : 0u;
           }
-          for (;;) {
-            bool any = false;
+          lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f);
 #pragma unroll
-            for (uint32_t j = 0; j < kIlp; ++j) any |= n[j] > 1;
-            if (!any) break;
-#pragma unroll
-            for (uint32_t j = 0; j < kIlp; ++j) {
-              if (n[j] > 1) {
-                const uint32_t half = n[j] >> 1;
-                const uint32_t v = Q.ci.cn_adj[lo[j] + half];
-                if (v <= w[j]) lo[j] += half;
-                n[j] -= half;
-              }
-            }
-          }
-#pragma unroll
-          for (uint32_t j = 0; j < kIlp; ++j) {
-            if (ok[j] && sm.drv[ri[j]] != x)
-              ok[j] = sm.alen[ri[j]][x] != 0 && Q.ci.cn_adj[lo[j]] == w[j];
-          }
+          for (uint32_t j = 0; j < kIlp; ++j)
+            if (ok[j] && sm.drv[ri[j]] != x) ok[j] = f[j];
         } else {
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j)
@@ -565,22 +666,47 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         }
       }
       if (count_level) {
-        if (Q.fused) {
-          const uint32_t pk = sm.lv_last.parent;
+        // the last position: |N(x) ∩ C(u_last)| for its parent's id x, minus
+        // the list members already in the row (same label, so injectivity)
+        if (fused_rows) {
+          uint64_t lo[kIlp];
+          uint32_t n[kIlp];
+          bool f[kIlp];
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j) {
-            if (!ok[j]) continue;
-            const uint32_t *row = s_rows + ri[j] * k;
-            const uint32_t xv = pk == k ? w[j] : row[pk];
-            uint64_t lb, le;
-            Q.ci.list(sm.lv_last, 0, xv, lb, le);
-            long long c = static_cast<long long>(le - lb);
-            for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) {
-              const uint32_t col = sm.lv_last.inj[x];
-              const uint32_t v = col == k ? w[j] : row[col];
-              if (v != xv && bit_of(Q.cand_last, v) && a.probe.has(xv, v)) --c;
+            contrib += ok[j] ? sm.fbase[ri[j]] : 0u;
+            const bool act = ok[j] && last_inj_k && w[j] != s_rows[ri[j] * k + sm.lv_last.parent];
+            lo[j] = sm.fbeg[ri[j]];
+            n[j] = act ? sm.flen[ri[j]] : 0u;
+          }
+          if (last_inj_k) {
+            lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f);
+#pragma unroll
+            for (uint32_t j = 0; j < kIlp; ++j) contrib -= f[j] ? 1u : 0u;
+          }
+        } else if (Q.fused) {  // the parent is this position: the list of w
+          uint64_t lb[kIlp], le[kIlp];
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            lb[j] = le[j] = 0;
+            if (ok[j]) Q.ci.list(sm.lv_last, 0, w[j], lb[j], le[j]);
+            contrib += le[j] - lb[j];
+          }
+          for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) {
+            const uint32_t col = sm.lv_last.inj[x];
+            if (col == k) continue;
+            uint64_t lo[kIlp];
+            uint32_t n[kIlp], v[kIlp];
+            bool f[kIlp];
+#pragma unroll
+            for (uint32_t j = 0; j < kIlp; ++j) {
+              v[j] = s_rows[ri[j] * k + col];
+              lo[j] = lb[j];
+              n[j] = ok[j] && v[j] != w[j] ? static_cast<uint32_t>(le[j] - lb[j]) : 0u;
             }
-            contrib += static_cast<unsigned long long>(c);
+            lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, v, f);
+#pragma unroll
+            for (uint32_t j = 0; j < kIlp; ++j) contrib -= f[j] ? 1u : 0u;
           }
         } else {
 #pragma unroll
@@ -631,22 +757,36 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         fill = 0;
         if (sm.flag) break;
       }
-      if (wcnt) {
-        uint32_t *blk = a.pool + static_cast<uint64_t>(sm.out_block) * a.block_words;
-        uint32_t pos = fill + wpos;
+      if (tile) {
+        // survivors (row, new id) to LDS in output order, then the tile's
+        // rows leave as one contiguous, coalesced run of words
+        if (wcnt) {
+          uint32_t pos = wpos;
 #pragma unroll
-        for (uint32_t j = 0; j < kIlp; ++j) {
-          if (ok[j]) {
-            uint32_t *dst = blk + static_cast<uint64_t>(pos + lanes_below(m[j])) * kout;
-            const uint32_t *row = s_rows + ri[j] * k;
-            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
-            dst[k] = w[j];
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            if (ok[j]) {
+              const uint32_t p = pos + lanes_below(m[j]);
+              sm.out_ri[p] = static_cast<uint8_t>(ri[j]);
+              sm.out_w[p] = w[j];
+            }
+            pos += static_cast<uint32_t>(__popcll(m[j]));
           }
-          pos += static_cast<uint32_t>(__popcll(m[j]));
+        }
+        __syncthreads();
+        uint32_t *dst = a.pool + static_cast<uint64_t>(sm.out_block) * a.block_words + static_cast<uint64_t>(fill) * kout;
+        const uint32_t nw = tile * kout;
+        for (uint32_t i = tid; i < nw; i += kThreads) {
+          const uint32_t r = __umulhi(i, kout_magic), c = i - r * kout;
+          dst[i] = c < k ? s_rows[sm.out_ri[r] * k + c] : sm.out_w[r];
         }
       }
       rows_written += tid == 0 ? tile : 0;
       fill += tile;
+    }
+    if (tid == 0) {
+      const uint64_t tn = __builtin_amdgcn_s_memrealtime();
+      sm.phase[count_level ? 1 : 2] += tn - tp0;
+      tp0 = tn;
     }
     if (sm.flag) break;
     __syncthreads();
@@ -682,7 +822,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       }
     }
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-    deque_push(a, wg, sm.push, np);
+    deque_push(a, sm, wg, sm.push, np);
     sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
     if (sm.acc_q != t.q) {
       flush_acc(a, sm);
@@ -694,6 +834,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     if (sm.acc_count >= (Q.cap >> 8) + 1 || sm.acc_items >= (1ull << 22)) flush_acc(a, sm);
     n_items += tot_items;
     n_rows_out += rows_written;
+    sm.phase[3] += __builtin_amdgcn_s_memrealtime() - tp0;
   }
   __syncthreads();
 }
@@ -705,7 +846,10 @@ __device__ inline uint32_t xorshift(uint32_t &s) {
   return s;
 }
 
-__global__ __launch_bounds__(kThreads) void batch_join_kernel(BArgs a) {
+// MINB: workgroups per CU the register allocation must allow (2 keeps every
+// value in registers; 4 caps them at 128 with a few spills).
+template <int MINB>
+__global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
   extern __shared__ uint32_t s_rows[];  // kRowBatch x widest row
   __shared__ Smem sm;
   const int tid = threadIdx.x;
@@ -717,13 +861,20 @@ __global__ __launch_bounds__(kThreads) void batch_join_kernel(BArgs a) {
   uint32_t rng = 0x9E3779B9u ^ (wg * 0x85EBCA6Bu + 1u);
   bool idle = false;  // counted in ctl->idle
   uint32_t idle_rounds = 0;
+  {  // no stale descriptor is ever read as a loop bound
+    uint32_t *z = reinterpret_cast<uint32_t *>(&sm.lv_last);
+    for (uint32_t i = tid; i < sizeof(LevelJoin) / 4; i += kThreads) z[i] = 0;
+  }
   if (tid == 0) {
     sm.staged_q = kNil;
     sm.staged_k = kNil;
     sm.ncache = 0;
+    sm.dq_bottom = sm.dq_pub = 0;
     sm.acc_q = kNil;
     sm.acc_count = sm.acc_items = sm.acc_rows = 0;
     sm.sync_ticks = 0;
+    atomicAdd(&a.ctl->started, 1ull);
+    sm.phase[0] = sm.phase[1] = sm.phase[2] = sm.phase[3] = 0;
   }
   for (;;) {
     if (tid == 0) {
@@ -731,7 +882,7 @@ __global__ __launch_bounds__(kThreads) void batch_join_kernel(BArgs a) {
       sm.state = 1;
       const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
       // own deque, then a new root task, then the oldest task of another workgroup
-      uint32_t rec = idle ? kNil : deque_pop(a, wg);  // an idle workgroup's deque is empty
+      uint32_t rec = idle ? kNil : deque_pop(a, sm, wg);  // an idle workgroup's deque is empty
       bool root = false;
       if (rec == kNil && ld_rlx(&a.ctl->root_next) < nroot) {
         if (idle) {
@@ -770,7 +921,14 @@ __global__ __launch_bounds__(kThreads) void batch_join_kernel(BArgs a) {
           atomicAdd(&a.ctl->idle, 1ull);
           idle = true;
         }
-        if (aborted(a) || (ld_rlx(&a.ctl->root_next) >= nroot && ld_rlx(&a.ctl->idle) == nwgs)) sm.state = 2;
+        // done when every workgroup that has begun is idle (idle read first: it
+        // never exceeds `started`, so equality held at the moment it was read)
+        if (aborted(a)) {
+          sm.state = 2;
+        } else if (ld_rlx(&a.ctl->root_next) >= nroot) {
+          const unsigned long long nidle = ld_rlx(&a.ctl->idle);
+          if (nidle == ld_rlx(&a.ctl->started)) sm.state = 2;
+        }
       }
       sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
     }
@@ -826,6 +984,7 @@ __global__ __launch_bounds__(kThreads) void batch_join_kernel(BArgs a) {
     atomicAdd(&a.ctl->busy_ticks, static_cast<unsigned long long>(busy));
     atomicAdd(&a.ctl->idle_ticks, static_cast<unsigned long long>(idle_t));
     atomicAdd(&a.ctl->sync_ticks, sm.sync_ticks);
+    for (int i = 0; i < 4; ++i) atomicAdd(&a.ctl->phase_ticks[i], sm.phase[i]);
     atomicAdd(&a.ctl->items, n_items);
     atomicAdd(&a.ctl->rows_in, n_rows_in);
     atomicAdd(&a.ctl->rows_out, n_rows_out);
@@ -877,7 +1036,7 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   {
     std::vector<FilterQ> fq(nqs);
     std::vector<const csm_query *> cq(queries.begin(), queries.end());
-    for (uint32_t i = 0; i < nqs; ++i) fq[i] = filter_entry(queries[i], stats + static_cast<uint64_t>(i) * kStatSize);
+    for (uint32_t i = 0; i < nqs; ++i) fq[i] = filter_entry(g, queries[i], stats + static_cast<uint64_t>(i) * kStatSize);
     CSM_HIP_TRY(ctx, hipMemcpyAsync(d_fq, fq.data(), sizeof(FilterQ) * nqs, hipMemcpyHostToDevice, s));
     const std::vector<uint32_t> steps = o.refine ? step_members_of(cq) : std::vector<uint32_t>();
     if (o.use_graph) {
@@ -954,13 +1113,11 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_bq, bq.data(), sizeof(BQuery) * nqs, hipMemcpyHostToDevice, s));
 
   // 4. queues and pool
-  TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 2u, 16u, 128u, kIlp, 4u};
-  uint64_t budget = 2000000000ull;
-  if (const char *e = std::getenv("CSM_BATCH_BUDGET")) budget = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
-  if (const char *e = std::getenv("CSM_BATCH_MAX_OUT")) tp.max_out = std::max(1, std::atoi(e));
-  if (const char *e = std::getenv("CSM_BATCH_ILP")) tp.ilp = std::min<uint32_t>(kIlp, std::max(1, std::atoi(e)));
-  if (const char *e = std::getenv("CSM_BATCH_BLOCK_KB")) tp.block_kb = std::max(32, std::atoi(e));
+  TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, o.block_kb ? o.block_kb : 128u,
+                o.ilp ? std::min(o.ilp, kIlp) : kIlp, o.max_out ? std::min(o.max_out, kMaxOut) : 1u};
+  const uint64_t budget = o.item_budget ? o.item_budget : 2000000000ull;
   if (o.chunk_rows) tp.root_words = std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, o.chunk_rows / 64));
+  if (tp.wgs_per_cu > 8 || tp.block_kb < 32 || tp.block_kb > 4096) return set_error(ctx, CSM_E_ARG, "batch tuning out of range");
   const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;
   if (block_words < static_cast<uint64_t>(kThreads) * CSM_MAX_QV) return set_error(ctx, CSM_E_ARG, "block too small");
   uint64_t pool_bytes = o.pool_bytes;
@@ -1004,13 +1161,13 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   a.root_words = tp.root_words;
   a.ilp = tp.ilp;
   a.max_out = std::min(tp.max_out, kMaxOut);
-  double limit_s = 120;
-  if (const char *e = std::getenv("CSM_BATCH_TIME_LIMIT")) limit_s = std::max(1.0, std::atof(e));
+  const double limit_s = o.time_limit_ms ? o.time_limit_ms * 1e-3 : 120.0;
   a.deadline_ticks = static_cast<uint64_t>(limit_s * 1e8);  // 100 MHz real-time clock
   const size_t lds_rows = 4ull * kRowBatch * std::max<uint32_t>(1, max_nq - 1);
   auto launch_join = [&]() {
     batch_init_kernel<<<grid_for(std::max<uint64_t>(num_blocks, grid)), 256, 0, s>>>(a, grid);
-    batch_join_kernel<<<grid, kThreads, lds_rows, s>>>(a);
+    if (tp.wgs_per_cu >= 4) batch_join_kernel<4><<<grid, kThreads, lds_rows, s>>>(a);
+    else batch_join_kernel<2><<<grid, kThreads, lds_rows, s>>>(a);
   };
   if (o.use_graph) {
     hipGraph_t graph;
@@ -1049,6 +1206,10 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
                  nqs, grid, static_cast<unsigned long long>(num_blocks), hctl.tasks, hctl.items, hctl.rows_in,
                  hctl.rows_out, hctl.steals, hctl.abort, hctl.overflow, hctl.fails, hctl.busy_ticks / 1e5 / grid,
                  hctl.idle_ticks / 1e5 / grid, hctl.sync_ticks / 1e5 / grid);
+  if (trace)
+    std::fprintf(stderr, "csm batch: per workgroup: staging %.1f ms, count tiles %.1f ms, extension tiles %.1f ms, task ends %.1f ms\n",
+                 hctl.phase_ticks[0] / 1e5 / grid, hctl.phase_ticks[1] / 1e5 / grid, hctl.phase_ticks[2] / 1e5 / grid,
+                 hctl.phase_ticks[3] / 1e5 / grid);
   const double ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (uint32_t i = 0; i < nqs; ++i) {
     csm_result &r = results[i];
@@ -1062,6 +1223,7 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
     r.ms_join = ms_total - ms_filter;
     r.items = hst[i].items;
     r.partial_rows = hst[i].rows;
+    r.path = bq[i].active ? CSM_PATH_BATCH : CSM_PATH_HOST;
     if (i == 0) {
       for (int c = 0; c < CSM_K_NUM; ++c) r.kernels[c] = kstat[c];
       r.kernels[CSM_K_BATCH].launches = 1;
@@ -1082,6 +1244,9 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
       if (st != CSM_OK) return st;
       results[i].count = r.count;
       results[i].capped = r.capped;
+      results[i].items += r.items;
+      results[i].partial_rows += r.partial_rows;
+      results[i].path = hst[i].done == kAbandoned ? CSM_PATH_BATCH_BUDGET : CSM_PATH_BATCH_ABORT;
       ++fallbacks;
     }
   }
@@ -1102,8 +1267,7 @@ extern "C" csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_quer
                                       const csm_batch_opts *opts, csm_result *out) {
   if (!ctx || !g || (n && (!qs || !out))) return CSM_E_ARG;
   csm_batch_opts o{};
-  o.refine = 1;
-  o.use_graph = 1;
+  o.use_graph = 1;  // refinement off by default: the candidate lists already prune dead ends
   if (opts) o = *opts;
   std::vector<csm_query *> v(qs, qs + n);
   for (auto *q : v)

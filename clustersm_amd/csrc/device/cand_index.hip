@@ -3,9 +3,7 @@
 // Two passes over the candidates of every slot (query edge s -> t): count
 // |N(x) ∩ C(t)| into the slot's offset array at x's rank, one exclusive scan
 // over all slots at once (offsets become absolute positions in cn_adj), then
-// write the lists.  A wave handles one 64-bit word of C(s); vertices above
-// kLightDegree are scanned by the whole wave, compacting in order with a
-// ballot, so every list stays sorted.
+// write the lists, each sorted.
 #include <algorithm>
 
 #include "cand_index.hpp"
@@ -16,7 +14,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
-constexpr uint32_t kLightDegree = 64;
 
 struct SlotDesc {
   const uint64_t *bits;       // the slot's query
@@ -46,10 +43,22 @@ __global__ void rank_words_kernel(const RankJob *jobs, uint32_t words) {
 
 __device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
 
+// A wave takes one 64-bit word of C(s): the adjacency ranges of its set
+// lanes are contiguous in the CSR order of the concatenation [lane 0's
+// neighbours, lane 1's, ...], which the wave walks 64 entries at a time
+// (coalesced loads, no divergence on skewed degrees).  Each entry finds its
+// owner lane by binary search over the degree prefix.  COUNT: hits per
+// owner, LDS atomics.  FILL: an owner's hits are written at its list start
+// plus the hits before them (earlier chunks: a running count; this chunk:
+// a ballot restricted to the owner's lanes), so every list stays sorted.
 template <bool FILL>
 __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                                                       uint32_t words, const SlotDesc *__restrict__ slots,
                                                       uint64_t *__restrict__ cn_off, uint32_t *__restrict__ cn_adj) {
+  __shared__ uint32_t s_pre[kWaves][kWave + 1];  // exclusive prefix of the lanes' degrees (0 for unset lanes)
+  __shared__ uint32_t s_o0[kWaves][kWave];       // adjacency start per lane
+  __shared__ uint32_t s_run[kWaves][kWave];      // hits so far per lane
+  __shared__ uint64_t s_start[kWaves][kWave];    // FILL: list start per lane
   const SlotDesc sd = slots[blockIdx.y];
   const uint64_t *bits = sd.bits;
   const uint64_t *rank_pref = sd.rank_pref;
@@ -58,6 +67,7 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
   const uint64_t *yrow = bits + static_cast<uint64_t>(sd.dst) * words;
   const uint64_t *pref = rank_pref + static_cast<uint64_t>(sd.src) * (words + 1);
   const uint64_t pref0 = pref[0];
+  uint32_t *pre = s_pre[wave];
   if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cn_off[sd.base + sd.ncand] = 0;
   for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {
     const uint64_t xw = xrow[b];
@@ -70,50 +80,46 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
       o0 = off[v];
       dv = off[v + 1] - o0;
     }
-    const bool light = act && dv <= kLightDegree;
-    if (!FILL) {
-      uint32_t c = 0;
-      if (light)
-        for (uint32_t o = o0; o < o0 + dv; ++o) c += bit_of(yrow, adj[o]);
-      uint64_t heavy = ballot(act && !light);
-      while (heavy) {
-        const int h = __builtin_ctzll(heavy);
-        heavy &= heavy - 1;
-        const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
-        uint32_t part = 0;
-        for (uint32_t o = ho + lane; o < ho + hd; o += kWave) part += bit_of(yrow, adj[o]);
-        for (int s = kWave / 2; s > 0; s >>= 1) part += __shfl_xor(part, s, kWave);
-        if (lane == h) c = part;
-      }
-      if (act) cn_off[slot] = c;
-    } else {
-      const uint64_t start = act ? cn_off[slot] : 0;
-      if (light) {
-        uint64_t p = start;
-        for (uint32_t o = o0; o < o0 + dv; ++o) {
-          const uint32_t w = adj[o];
-          if (bit_of(yrow, w)) cn_adj[p++] = w;
-        }
-      }
-      uint64_t heavy = ballot(act && !light);
-      while (heavy) {
-        const int h = __builtin_ctzll(heavy);
-        heavy &= heavy - 1;
-        const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
-        uint64_t p = __shfl(start, h, kWave);
-        for (uint32_t o = ho; o < ho + hd; o += kWave) {
-          bool f = false;
-          uint32_t w = 0;
-          if (o + lane < ho + hd) {
-            w = adj[o + lane];
-            f = bit_of(yrow, w);
-          }
-          const uint64_t m = ballot(f);
-          if (f) cn_adj[p + lanes_below(m)] = w;
-          p += static_cast<uint64_t>(__popcll(m));
-        }
-      }
+    uint32_t incl = dv;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t y = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += y;
     }
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    pre[lane] = incl - dv;
+    if (lane == kWave - 1) pre[kWave] = total;
+    s_o0[wave][lane] = o0;
+    s_run[wave][lane] = 0;
+    if (FILL) s_start[wave][lane] = act ? cn_off[slot] : 0;
+    wave_sync();
+    for (uint32_t base = 0; base < total; base += kWave) {
+      const uint32_t e = base + static_cast<uint32_t>(lane);
+      bool hit = false;
+      uint32_t owner = 0, w = 0;
+      if (e < total) {
+        uint32_t lo = 0, hi = kWave;  // last lane with pre <= e (unset lanes have empty ranges)
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (pre[mid] <= e) lo = mid; else hi = mid;
+        }
+        owner = lo;
+        w = adj[s_o0[wave][owner] + (e - pre[owner])];
+        hit = bit_of(yrow, w);
+      }
+      if (FILL) {
+        const uint64_t m = ballot(hit);
+        if (hit) {
+          const uint32_t first = max(base, pre[owner]) - base;  // owner's first lane in this chunk
+          const uint64_t seg = m & ~((1ull << first) - 1ull);
+          cn_adj[s_start[wave][owner] + s_run[wave][owner] + lanes_below(seg)] = w;
+        }
+        wave_sync();
+      }
+      if (hit) atomicAdd(&s_run[wave][owner], 1u);
+      wave_sync();
+    }
+    if (!FILL && act) cn_off[slot] = s_run[wave][lane];
+    wave_sync();
   }
 }
 

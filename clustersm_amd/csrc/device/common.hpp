@@ -74,9 +74,15 @@ struct csm_graph {
   uint64_t *hub_bits = nullptr;  // num_hubs x ceil(n / 64)
   uint32_t num_hubs = 0;
   uint32_t hub_min_degree = 0;
+  // Neighbour-label counts per vertex, saturating at 255: row v holds
+  // kNlcStride bytes, byte l = |{w in N(v) : label(w) = l}| for l < nlc_labels.
+  uint8_t *nlc = nullptr;
+  uint32_t nlc_labels = 0;
 };
 
 namespace csm {
+
+constexpr uint32_t kNlcStride = 64;  // labels covered by csm_graph::nlc
 
 // Per-query descriptor, passed to kernels by value.  Replaces the reference's
 // __constant__ globals (globals.cu:25-38) so several queries can be resident.

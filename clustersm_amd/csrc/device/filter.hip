@@ -48,6 +48,7 @@ __global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(const uint32
                                                                     const FilterQ *__restrict__ qs) {
   __shared__ FilterSmem sm;
   const FilterQ fq = qs[blockIdx.y];
+  if (fq.use_table) return;
   const int tid = threadIdx.x;
   {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(fq.desc);
@@ -120,6 +121,70 @@ __global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(const uint32
   if (lane == 0 && scanned) atomicAdd(&s_scan, static_cast<unsigned long long>(scanned));
   __syncthreads();
   if (tid == 0 && s_scan) atomicAdd(&fq.stats[kStatFilterScanned], s_scan);
+  if (tid < static_cast<int>(nq) && sm.cnt[tid]) atomicAdd(&fq.stats[tid], sm.cnt[tid]);
+}
+
+// The same predicate read from the graph's neighbour-label table: one wave
+// per 64 vertices, each lane reads its vertex's label and degree and, when
+// they can match, the bytes of its table row that the signatures name.
+// A coalesced stream instead of an adjacency scan with label gathers.
+__global__ __launch_bounds__(kFilterThreads) void nlc_table_filter_kernel(const uint32_t *__restrict__ off,
+                                                                          const uint32_t *__restrict__ label,
+                                                                          const uint8_t *__restrict__ nlc, uint32_t n,
+                                                                          const FilterQ *__restrict__ qs) {
+  __shared__ FilterSmem sm;
+  const FilterQ fq = qs[blockIdx.y];
+  if (!fq.use_table) return;
+  const int tid = threadIdx.x;
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(fq.desc);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.d);
+    for (uint32_t i = tid; i < sizeof(QueryDesc) / 4; i += kFilterThreads) dst[i] = src[i];
+    for (uint32_t i = tid; i < kLabelTable; i += kFilterThreads) sm.slot_of[i] = kNoSlot;
+    if (tid < CSM_MAX_QV) sm.cnt[tid] = 0;
+  }
+  __syncthreads();
+  if (tid < static_cast<int>(sm.d.num_slots) && sm.d.slot_label[tid] < kLabelTable)
+    sm.slot_of[sm.d.slot_label[tid]] = static_cast<uint8_t>(tid);
+  __syncthreads();
+  const int wave = tid / kWave, lane = tid % kWave;
+  const uint32_t words = sm.d.words, nq = sm.d.nq;
+  uint64_t *bits = fq.bits;
+  uint64_t rows = 0;
+  for (uint32_t b = blockIdx.x * kFilterWaves + wave; b < words; b += gridDim.x * kFilterWaves) {
+    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
+    uint32_t s = kNoSlot, dv = 0;
+    if (v < n) {
+      s = lookup_slot(sm, label[v]);
+      dv = off[v + 1] - off[v];
+    }
+    const bool active = s != kNoSlot && dv >= sm.d.slot_min_deg[s];
+    rows += active ? 1 : 0;
+    const uint8_t *row = nlc + static_cast<uint64_t>(v) * kNlcStride;
+    for (uint32_t u = 0; u < nq; ++u) {
+      bool ok = active && sm.d.slot[u] == s && sm.d.deg[u] <= dv;
+      if (ok)
+        for (uint32_t t = sm.d.sig_off[u]; t < sm.d.sig_off[u + 1]; ++t)
+          if (row[sm.d.slot_label[sm.d.sig_slot[t]]] < sm.d.sig_cnt[t]) {
+            ok = false;
+            break;
+          }
+      const uint64_t m = ballot(ok);
+      if (lane == 0) {
+        bits[static_cast<uint64_t>(u) * words + b] = m;
+        if (m) atomicAdd(&sm.cnt[u], static_cast<unsigned long long>(__popcll(m)));
+      }
+    }
+  }
+  // one table row (kNlcStride bytes) per vertex that could match, counted as
+  // 8-byte units in the scanned statistic
+  for (int d = kWave / 2; d > 0; d >>= 1) rows += __shfl_xor(rows, d, kWave);
+  __shared__ unsigned long long s_rows;
+  if (tid == 0) s_rows = 0;
+  __syncthreads();
+  if (lane == 0 && rows) atomicAdd(&s_rows, static_cast<unsigned long long>(rows) * (kNlcStride / 8));
+  __syncthreads();
+  if (tid == 0 && s_rows) atomicAdd(&fq.stats[kStatFilterScanned], s_rows);
   if (tid < static_cast<int>(nq) && sm.cnt[tid]) atomicAdd(&fq.stats[tid], sm.cnt[tid]);
 }
 
@@ -233,6 +298,8 @@ void launch_filter_batch(hipStream_t s, const csm_graph *g, const FilterQ *d_qs,
   zero_stats_kernel<<<nqueries, 64, 0, s>>>(d_qs, nqueries, 0);
   if (kt) kt->begin(CSM_K_FILTER);
   nlc_filter_kernel<<<dim3(gx, nqueries), kFilterThreads, 0, s>>>(g->off, g->adj, g->label, g->n, d_qs);
+  if (g->nlc)
+    nlc_table_filter_kernel<<<dim3(gx, nqueries), kFilterThreads, 0, s>>>(g->off, g->label, g->nlc, g->n, d_qs);
   if (kt) kt->end(CSM_K_FILTER, 0);
   if (!refine) return;
   for (uint32_t c = 0; c < step_members.size(); ++c) {
@@ -263,8 +330,11 @@ std::vector<uint32_t> step_members_of(const std::vector<const csm_query *> &qs) 
   return m;
 }
 
-FilterQ filter_entry(csm_query *q, uint64_t *stats) {
+FilterQ filter_entry(const csm_graph *g, csm_query *q, uint64_t *stats) {
   FilterQ f{};
+  bool covered = g->nlc != nullptr;
+  for (uint32_t u = 0; u < q->desc.nq; ++u) covered = covered && q->desc.label[u] < g->nlc_labels;
+  f.use_table = covered ? 1u : 0u;
   f.desc = q->d_desc;
   f.bits = q->bits;
   f.cl_us = q->d_cluster_us;
@@ -287,7 +357,7 @@ csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine
     ctx->arena.free(d_q);
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (filter)");
   }
-  FilterQ fq = filter_entry(q, stats);
+  FilterQ fq = filter_entry(g, q, stats);
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_q, &fq, sizeof(FilterQ), hipMemcpyHostToDevice, s));
   std::memset(q->kstat, 0, sizeof(q->kstat));
   {

@@ -16,9 +16,11 @@ struct FilterQ {
   uint32_t num_clusters;
   uint32_t nq;
   unsigned long long *stats;  // kStatSize entries
+  uint32_t use_table;         // every query label is covered by the graph's neighbour-label table
+  uint32_t pad;
 };
 
-FilterQ filter_entry(csm_query *q, uint64_t *stats);
+FilterQ filter_entry(const csm_graph *g, csm_query *q, uint64_t *stats);
 
 // Largest member count of cluster c over the queries, per refinement step c.
 std::vector<uint32_t> step_members_of(const std::vector<const csm_query *> &qs);

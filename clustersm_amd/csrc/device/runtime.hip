@@ -151,6 +151,64 @@ __global__ void hub_bits_kernel(const uint32_t *off, const uint32_t *adj, const 
 
 }  // namespace
 
+// Neighbour-label table: one wave per 64 vertices, a byte histogram per lane
+// in LDS (saturating), heavy vertices counted by the whole wave.
+__global__ __launch_bounds__(256) void nlc_table_kernel(const uint32_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ adj,
+                                                         const uint32_t *__restrict__ label, uint32_t n,
+                                                         uint32_t nlabels, uint8_t *__restrict__ nlc) {
+  __shared__ uint32_t hist[4][kNlcStride / 4][kWave];  // [wave][label / 4][lane], 4 byte counters per word
+  __shared__ uint32_t heavy[4][kNlcStride];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint32_t words = (n + 63) / 64;
+  for (uint32_t b = blockIdx.x * 4 + wave; b < words; b += gridDim.x * 4) {
+    const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
+    uint32_t o0 = 0, dv = 0;
+    if (v < n) {
+      o0 = off[v];
+      dv = off[v + 1] - o0;
+    }
+    for (uint32_t i = 0; i < kNlcStride / 4; ++i) hist[wave][i][lane] = 0;
+    const bool light = dv <= 64;
+    if (light) {
+      for (uint32_t o = o0; o < o0 + dv; ++o) {
+        const uint32_t l = label[adj[o]];
+        if (l < nlabels) {
+          const uint32_t sh = (l & 3) * 8, cur = (hist[wave][l >> 2][lane] >> sh) & 0xFF;
+          if (cur < 255) hist[wave][l >> 2][lane] += 1u << sh;
+        }
+      }
+    }
+    uint64_t hv = ballot(!light);
+    while (hv) {
+      const int h = __builtin_ctzll(hv);
+      hv &= hv - 1;
+      const uint32_t ho = __shfl(o0, h, kWave), hd = __shfl(dv, h, kWave);
+      heavy[wave][lane] = 0;
+      wave_sync();
+      for (uint32_t o = ho + lane; o < ho + hd; o += kWave) {
+        const uint32_t l = label[adj[o]];
+        if (l < nlabels) atomicAdd(&heavy[wave][l], 1u);
+      }
+      wave_sync();
+      if (lane < static_cast<int>(kNlcStride / 4)) {
+        uint32_t wv = 0;
+        for (int j = 0; j < 4; ++j) wv |= min(heavy[wave][lane * 4 + j], 255u) << (8 * j);
+        hist[wave][lane][h] = wv;
+      }
+      wave_sync();
+    }
+    wave_sync();
+    if (v < n) {
+      uint4 *row = reinterpret_cast<uint4 *>(nlc + static_cast<uint64_t>(v) * kNlcStride);
+      for (uint32_t i = 0; i < kNlcStride / 16; ++i)
+        row[i] = make_uint4(hist[wave][4 * i][lane], hist[wave][4 * i + 1][lane], hist[wave][4 * i + 2][lane],
+                            hist[wave][4 * i + 3][lane]);
+    }
+    wave_sync();
+  }
+}
+
 // Hubs: vertices of degree >= kHubMinDegree, highest degree first, while the
 // bitmaps fit in the budget (1/16 of the arena, at most 4 GiB).
 csm_status build_hubs(csm_ctx *ctx, csm_graph *g) {
@@ -196,6 +254,18 @@ csm_status build_hubs(csm_ctx *ctx, csm_graph *g) {
   return CSM_OK;
 }
 
+csm_status build_nlc(csm_ctx *ctx, csm_graph *g) {
+  g->nlc_labels = std::min<uint32_t>(kNlcStride, g->max_label + 1);
+  g->nlc = static_cast<uint8_t *>(ctx->arena.alloc(static_cast<uint64_t>(kNlcStride) * std::max<uint32_t>(g->n, 1)));
+  if (!g->nlc) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (neighbour-label table)");
+  if (g->n) {
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(65535, (ceil_div(g->n, 64) + 3) / 4));
+    nlc_table_kernel<<<blocks, 256, 0, ctx->stream>>>(g->off, g->adj, g->label, g->n, g->nlc_labels, g->nlc);
+  }
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return CSM_OK;
+}
+
 csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels, uint64_t m,
                                const uint32_t *edges, csm_graph **out) {
   hipStream_t s = ctx->stream;
@@ -229,6 +299,7 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
     g->adj = static_cast<uint32_t *>(ctx->arena.alloc(4));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     csm_status hst = build_hubs(ctx, g.get());
+    if (hst == CSM_OK) hst = build_nlc(ctx, g.get());
     if (hst != CSM_OK) return hst;
     *out = g.release();
     return CSM_OK;
@@ -295,6 +366,7 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
   g->m = total / 2;
   g->max_degree = maxdeg;
   csm_status hst = build_hubs(ctx, g.get());
+  if (hst == CSM_OK) hst = build_nlc(ctx, g.get());
   if (hst != CSM_OK) return fail(hst, ctx->last_error);
   *out = g.release();
   return CSM_OK;
@@ -454,6 +526,7 @@ csm_status csm_graph_free(csm_ctx *ctx, csm_graph *g) {
   ctx->arena.free(g->label);
   ctx->arena.free(g->hub_id);
   ctx->arena.free(g->hub_bits);
+  ctx->arena.free(g->nlc);
   delete g;
   return CSM_OK;
 }

@@ -205,8 +205,19 @@ typedef struct csm_result {
   double ms_join;
   double ms_total;
   double ms_latency;       /* serving: arrival to result */
+  uint32_t path;           /* csm_path: which method produced the result */
+  uint32_t reserved;
   csm_kernel_stat kernels[CSM_K_NUM];
 } csm_result;
+
+/* csm_result.path */
+enum {
+  CSM_PATH_PLAIN = 0,       /* csm_match */
+  CSM_PATH_HOST = 1,        /* answered without a join (two vertices, or an empty candidate set) */
+  CSM_PATH_BATCH = 2,       /* the persistent batch kernel */
+  CSM_PATH_BATCH_BUDGET = 3,/* over the batch item budget: finished by the plain path */
+  CSM_PATH_BATCH_ABORT = 4  /* the batch kernel gave up (time limit, pool): finished by the plain path */
+};
 
 /* Per-launch device timing with HIP events on the context stream. */
 csm_status csm_set_profiling(csm_ctx *ctx, int on);
@@ -220,11 +231,17 @@ csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q,
  * device work queue of (query, level, row chunk) tasks.  out: n results. */
 typedef struct csm_batch_opts {
   uint64_t cap;            /* per-query cap, 0 = unlimited */
-  uint32_t refine;
+  uint32_t refine;         /* cluster refinement before the join (opts == NULL: off; the
+                              candidate-neighbour lists already end dead branches early) */
   uint32_t use_graph;      /* capture the launch sequence in a hipGraph */
-  uint32_t chunk_rows;     /* 0 = autotuned */
-  uint32_t wgs_per_cu;     /* 0 = autotuned */
+  uint32_t chunk_rows;     /* root ids per root task (multiple of 64), 0 = default 1024 */
+  uint32_t wgs_per_cu;     /* persistent workgroups per CU (1..8), 0 = default 4 */
   uint64_t pool_bytes;     /* frontier pool, 0 = auto */
+  uint32_t block_kb;       /* pool block size in KiB (32..4096), 0 = default 128 */
+  uint32_t max_out;        /* full output blocks a task holds before it yields (1..8), 0 = default 1 */
+  uint32_t ilp;            /* items per thread per tile (1..4), 0 = default 4 */
+  uint32_t time_limit_ms;  /* the persistent kernel gives up after this; the plain path finishes, 0 = 120 s */
+  uint64_t item_budget;    /* items per query before the plain path takes it over, 0 = 2e9 */
 } csm_batch_opts;
 csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
                            const csm_batch_opts *opts, csm_result *out);

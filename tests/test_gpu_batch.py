@@ -43,7 +43,9 @@ def test_batch_small_random_vs_oracle(ctx):
             want = _oracle_counts(dl, de, queries, cap)
             for use_graph in (True, False):
                 print("batch round", round_, "queries", len(qs), "cap", cap, "graph", use_graph, flush=True)
-                res = csm.batch_match(g, qs, cap=cap, use_graph=use_graph)
+                # a short time limit turns a scheduling fault into a reported abort
+                res = csm.batch_match(g, qs, cap=cap, use_graph=use_graph, time_limit_ms=20000)
+                assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in res), [x.path for x in res]
                 got = [(x.count, bool(x.capped)) for x in res]
                 assert got == want, (round_, cap, use_graph)
         for q in qs:
@@ -63,10 +65,11 @@ def test_batch_equals_plain_and_alone(ctx):
         plans.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 500 + i, avg_degree=r.uniform(2, 3.5)).plan())
     qs = [g.prepare(p) for p in plans]
     cap = 200_000
-    mix = csm.batch_match(g, qs, cap=cap)
+    mix = csm.batch_match(g, qs, cap=cap, time_limit_ms=20000)
+    assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in mix)
     for i, q in enumerate(qs):
         plain, _ = q.match(cap=cap)
-        alone = csm.batch_match(g, [q], cap=cap)[0]
+        alone = csm.batch_match(g, [q], cap=cap, time_limit_ms=20000)[0]
         print("query", i, plans[i].num_v, plain.count, mix[i].count, flush=True)
         assert (mix[i].count, mix[i].capped) == (plain.count, plain.capped) == (alone.count, alone.capped), i
     for q in qs:
@@ -75,7 +78,8 @@ def test_batch_equals_plain_and_alone(ctx):
 
 
 def test_batch_small_pool_continuations(ctx):
-    """A pool of a few blocks forces yields and, if it runs dry, the plain fallback."""
+    """A pool of a few blocks forces yields and, if it runs dry, the plain fallback.
+    Tuning variants (block size, output bound, lanes per item) give the same counts."""
     hg = csm.gen_rmat(11, 30000, 4, 0.5, 4)
     g = ctx.upload(hg)
     host = g.to_host()
@@ -84,8 +88,30 @@ def test_batch_small_pool_continuations(ctx):
     cap = 3_000_000
     want = [q.match(cap=cap)[0] for q in qs]
     for pool in (64 * 128 * 1024, 256 * 128 * 1024):
-        res = csm.batch_match(g, qs, cap=cap, pool_bytes=pool)
+        res = csm.batch_match(g, qs, cap=cap, pool_bytes=pool, time_limit_ms=20000)
         assert [(x.count, x.capped) for x in res] == [(w.count, w.capped) for w in want], pool
+    for q in qs:
+        q.free()
+    g.free()
+
+
+def test_batch_tuning_variants_agree(ctx):
+    hg = csm.gen_rmat(12, 40000, 6, 1.0, 21)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    r = random.Random(8)
+    plans = [csm.gen_query(host, r.randint(3, 14), csm.QUERY_AVG_DEGREE, 900 + i, avg_degree=r.uniform(2, 3.5)).plan()
+             for i in range(16)]
+    qs = [g.prepare(p) for p in plans]
+    cap = 500_000
+    want = [(x.count, x.capped) for x in (q.match(cap=cap)[0] for q in qs)]
+    for kw in (dict(wgs_per_cu=1), dict(wgs_per_cu=3, max_out=1), dict(block_kb=32, ilp=1), dict(max_out=8, ilp=2),
+               dict(chunk_rows=64), dict(item_budget=1000)):
+        print("tuning", kw, flush=True)
+        res = csm.batch_match(g, qs, cap=cap, time_limit_ms=20000, **kw)
+        assert [(x.count, x.capped) for x in res] == want, kw
+        if "item_budget" not in kw:
+            assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in res), kw
     for q in qs:
         q.free()
     g.free()

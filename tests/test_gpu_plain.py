@@ -125,6 +125,33 @@ def test_filter_and_refine_bitmaps_vs_oracle(ctx):
         g.free()
 
 
+def test_filter_labels_beyond_table_vs_oracle(ctx):
+    """Query labels >= 64 are not covered by the neighbour-label table: the
+    adjacency-scanning filter runs.  Both paths are checked in one graph."""
+    r = random.Random(12)
+    for it in range(8):
+        n = r.randint(100, 400)
+        nl = 90  # labels 0..89: some queries fall outside the table
+        dl, de = rand_graph(r, n, r.uniform(0.05, 0.2), 1)
+        dl = [r.choice((r.randint(0, 5), r.randint(60, 70))) for _ in dl]
+        d = O.Data.from_arrays(dl, de)
+        g = ctx.upload_edges(dl, de)
+        for hi in (False, True):
+            nq = r.randint(2, 6)
+            ql, qe = rand_query(r, nq, r.randint(0, nq), 1)
+            ql = [r.randint(60, 70) if hi else r.randint(0, 5) for _ in ql]
+            print("label case", it, hi, n, nq, flush=True)
+            oq = O.Query.from_arrays(ql, qe)
+            q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+            q.filter(refine=False)
+            assert (q.candidates() == O.filter_nlc(d, len(dl), oq)).all(), (it, hi)
+            m, _ = q.match(cap=0)
+            c, _, _, _, _ = O.match(d, oq, oq.plan(), cap=(1 << 64) - 1, threads=4)
+            assert m.count == c, (it, hi)
+            q.free()
+        g.free()
+
+
 def test_random_matches_vs_oracle_with_small_chunks(ctx):
     """Many chunks per level (DFS across chunks) must not change the result or its order."""
     r = random.Random(4)
