@@ -343,7 +343,10 @@ class Context:
         return DeviceGraph(self, h)
 
     def serve_start(self, graph, budget_bytes=64 << 30, max_inflight=64, queue_capacity=4096, cap=0,
-                    refine=True, chunk_rows=0):
+                    refine=False, chunk_rows=0):
+        """Continuous batching against `graph` (csm_serve_start); submit and
+        poll from any thread, and no other matching call on this context
+        until serve_stop."""
         o = ServeOpts(budget_bytes, max_inflight, queue_capacity, cap, 1 if refine else 0, chunk_rows)
         self.check(lib().csm_serve_start(self.h, graph.h, C.byref(o)), "csm_serve_start")
 

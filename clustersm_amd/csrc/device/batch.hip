@@ -33,7 +33,7 @@
 #include <cstdlib>
 #include <numeric>
 
-#include "cand_index.hpp"
+#include "batch.hpp"
 #include "filter.hpp"
 
 namespace csm {
@@ -1001,49 +1001,65 @@ struct TuneParams {
 
 }  // namespace
 
-// Runs the batch; results[i] for queries[i].  Queries must be prepared.
-csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_query *> &queries,
-                     const csm_batch_opts &o, csm_result *results) {
-  auto t0 = std::chrono::steady_clock::now();
-  hipStream_t s = ctx->stream;
-  const uint32_t nqs = static_cast<uint32_t>(queries.size());
-  if (nqs == 0) return CSM_OK;
-  const uint32_t words = queries[0]->desc.words;
-  const uint64_t cap = o.cap ? o.cap : ~0ull;
-  std::vector<void *> blocks;
-  struct Guard {
-    csm_ctx *ctx;
-    std::vector<void *> &b;
-    ~Guard() {
-      hipStreamSynchronize(ctx->stream);
-      for (void *p : b) ctx->arena.free(p);
-    }
-  } guard{ctx, blocks};
-  auto get = [&](uint64_t bytes) {
-    void *p = ctx->arena.alloc(bytes);
-    if (p) blocks.push_back(p);
-    return p;
-  };
-  csm_kernel_stat kstat[CSM_K_NUM] = {};
-  KernelTimer kt(ctx, kstat);
-  uint32_t max_nq = 0;
-  for (auto *q : queries) max_nq = std::max(max_nq, q->desc.nq);
+// Makes ctx->stream = s for the helpers that launch on the context stream
+// (and time with its events) while one stage runs.
+struct StreamSwap {
+  csm_ctx *ctx;
+  hipStream_t old;
+  StreamSwap(csm_ctx *c, hipStream_t s) : ctx(c), old(c->stream) { c->stream = s; }
+  ~StreamSwap() { ctx->stream = old; }
+};
 
-  // 1. filter + refinement + counts for every query (one launch per step)
-  uint64_t *stats = static_cast<uint64_t *>(get(8ull * kStatSize * nqs));
-  FilterQ *d_fq = static_cast<FilterQ *>(get(sizeof(FilterQ) * nqs));
+BatchRun::BatchRun(csm_ctx *ctx, const csm_graph *g, std::vector<csm_query *> queries, const csm_batch_opts &o)
+    : ctx_(ctx), g_(g), queries_(std::move(queries)), o_(o), kstat_(CSM_K_NUM) {
+  t0_ = std::chrono::steady_clock::now();
+  words_ = queries_.empty() ? 0 : queries_[0]->desc.words;
+  for (auto *q : queries_) max_nq_ = std::max(max_nq_, q->desc.nq);
+}
+
+void BatchRun::release_graph() {
+  if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+  if (graph_) hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+BatchRun::~BatchRun() {
+  if (join_stream_) hipStreamSynchronize(join_stream_);
+  release_graph();
+  join_kt_.reset();
+  hipStreamSynchronize(ctx_->stream);
+  for (void *p : blocks_) ctx_->arena.free(p);
+}
+
+void *BatchRun::get(uint64_t bytes) {
+  void *p = ctx_->arena.alloc(bytes);
+  if (p) blocks_.push_back(p);
+  return p;
+}
+
+csm_status BatchRun::prepare(hipStream_t s) {
+  StreamSwap swap(ctx_, s);
+  csm_ctx *ctx = ctx_;
+  const csm_graph *g = g_;
+  const uint32_t nqs = static_cast<uint32_t>(queries_.size()), words = words_;
+  const uint64_t cap = o_.cap ? o_.cap : ~0ull;
+  KernelTimer kt(ctx, kstat_.data());
+  // 1. filter (+ refinement) and candidate counts for every query, one launch per step
+  uint64_t *stats = static_cast<uint64_t *>(get(8ull * kStatSize * std::max<uint32_t>(nqs, 1)));
+  FilterQ *d_fq = static_cast<FilterQ *>(get(sizeof(FilterQ) * std::max<uint32_t>(nqs, 1)));
   if (!stats || !d_fq) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (batch filter)");
   {
     std::vector<FilterQ> fq(nqs);
-    std::vector<const csm_query *> cq(queries.begin(), queries.end());
-    for (uint32_t i = 0; i < nqs; ++i) fq[i] = filter_entry(g, queries[i], stats + static_cast<uint64_t>(i) * kStatSize);
+    std::vector<const csm_query *> cq(queries_.begin(), queries_.end());
+    for (uint32_t i = 0; i < nqs; ++i) fq[i] = filter_entry(g, queries_[i], stats + static_cast<uint64_t>(i) * kStatSize);
     CSM_HIP_TRY(ctx, hipMemcpyAsync(d_fq, fq.data(), sizeof(FilterQ) * nqs, hipMemcpyHostToDevice, s));
-    const std::vector<uint32_t> steps = o.refine ? step_members_of(cq) : std::vector<uint32_t>();
-    if (o.use_graph) {
+    const std::vector<uint32_t> steps = o_.refine ? step_members_of(cq) : std::vector<uint32_t>();
+    if (o_.use_graph) {
       hipGraph_t graph;
       hipGraphExec_t exec;
       CSM_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_filter_batch(s, g, d_fq, nqs, words, steps, max_nq, o.refine != 0, nullptr);
+      launch_filter_batch(s, g, d_fq, nqs, words, steps, max_nq_, o_.refine != 0, nullptr);
       CSM_HIP_TRY(ctx, hipStreamEndCapture(s, &graph));
       CSM_HIP_TRY(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       kt.begin(CSM_K_FILTER);
@@ -1053,42 +1069,42 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
       hipGraphExecDestroy(exec);
       hipGraphDestroy(graph);
     } else {
-      launch_filter_batch(s, g, d_fq, nqs, words, steps, max_nq, o.refine != 0, &kt);
+      launch_filter_batch(s, g, d_fq, nqs, words, steps, max_nq_, o_.refine != 0, &kt);
     }
     std::vector<uint64_t> host(static_cast<size_t>(kStatSize) * nqs);
     CSM_HIP_TRY(ctx, hipMemcpyAsync(host.data(), stats, 8ull * host.size(), hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     for (uint32_t i = 0; i < nqs; ++i) {
-      for (uint32_t u = 0; u < queries[i]->desc.nq; ++u) queries[i]->counts[u] = host[i * kStatSize + u];
-      queries[i]->filtered = true;
+      for (uint32_t u = 0; u < queries_[i]->desc.nq; ++u) queries_[i]->counts[u] = host[i * kStatSize + u];
+      queries_[i]->filtered = true;
     }
   }
-  const double ms_filter = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ms_filter_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
 
   // 2. candidate indexes of every query
-  std::vector<CandIndexHost> cih;
   {
-    std::vector<const csm_query *> cq(queries.begin(), queries.end());
-    csm_status st = build_cand_index_batch(ctx, g, cq, cih, blocks, kt);
+    std::vector<const csm_query *> cq(queries_.begin(), queries_.end());
+    csm_status st = build_cand_index_batch(ctx, g, cq, cih_, blocks_, kt);
     if (st != CSM_OK) return st;
   }
 
   // 3. per-query descriptors; queries answered without the join
   std::vector<BQuery> bq(nqs);
-  std::vector<uint64_t> host_count(nqs, 0);
-  LevelJoin *d_lv = static_cast<LevelJoin *>(get(sizeof(LevelJoin) * CSM_MAX_QV * nqs));
-  BQuery *d_bq = static_cast<BQuery *>(get(sizeof(BQuery) * nqs));
-  BState *d_st = static_cast<BState *>(get(sizeof(BState) * nqs));
-  if (!d_lv || !d_bq || !d_st) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (batch descriptors)");
+  host_count_.assign(nqs, 0);
+  active_.assign(nqs, 0);
+  LevelJoin *d_lv = static_cast<LevelJoin *>(get(sizeof(LevelJoin) * CSM_MAX_QV * std::max<uint32_t>(nqs, 1)));
+  d_bq_ = get(sizeof(BQuery) * std::max<uint32_t>(nqs, 1));
+  d_st_ = get(sizeof(BState) * std::max<uint32_t>(nqs, 1));
+  if (!d_lv || !d_bq_ || !d_st_) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (batch descriptors)");
   std::vector<LevelJoin> lvs(static_cast<size_t>(CSM_MAX_QV) * nqs);
-  std::vector<std::pair<uint64_t, uint64_t>> two_vertex;  // (query, cn_off index pair)
+  std::vector<std::pair<uint64_t, uint64_t>> two_vertex;  // (query, cn_off index of its one slot)
   for (uint32_t i = 0; i < nqs; ++i) {
-    const csm_query *q = queries[i];
+    const csm_query *q = queries_[i];
     const uint32_t nq = q->desc.nq, L = nq - 1;
-    for (uint32_t k = 0; k < nq; ++k) lvs[static_cast<size_t>(i) * CSM_MAX_QV + k] = cih[i].level[k];
+    for (uint32_t k = 0; k < nq; ++k) lvs[static_cast<size_t>(i) * CSM_MAX_QV + k] = cih_[i].level[k];
     BQuery &b = bq[i];
     std::memset(&b, 0, sizeof(b));
-    b.ci = CandIndex{q->bits, cih[i].rank_pref, cih[i].cn_off, cih[i].cn_adj, words};
+    b.ci = CandIndex{q->bits, cih_[i].rank_pref, cih_[i].cn_off, cih_[i].cn_adj, words};
     b.lv = d_lv + static_cast<size_t>(i) * CSM_MAX_QV;
     b.root_bits = q->bits + static_cast<uint64_t>(q->level[0].u) * words;
     b.cand_last = q->bits + static_cast<uint64_t>(q->level[L].u) * words;
@@ -1099,20 +1115,31 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
     b.active = q->counts[q->level[0].u] > 0;
     if (L == 1 && b.active) {  // two vertices: total length of the one slot
       b.active = 0;
-      two_vertex.push_back({i, cih[i].level[1].off_base[0]});
+      two_vertex.push_back({i, cih_[i].level[1].off_base[0]});
     }
+    active_[i] = static_cast<uint8_t>(b.active);
   }
   for (auto &tv : two_vertex) {
-    const uint64_t n1 = queries[tv.first]->counts[queries[tv.first]->level[0].u];
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, cih[tv.first].cn_off + tv.second, 8, hipMemcpyDeviceToHost, s));
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned + 1, cih[tv.first].cn_off + tv.second + n1, 8, hipMemcpyDeviceToHost, s));
+    const uint64_t n1 = queries_[tv.first]->counts[queries_[tv.first]->level[0].u];
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, cih_[tv.first].cn_off + tv.second, 8, hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned + 1, cih_[tv.first].cn_off + tv.second + n1, 8, hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-    host_count[tv.first] = ctx->pinned[1] - ctx->pinned[0];
+    host_count_[tv.first] = ctx->pinned[1] - ctx->pinned[0];
   }
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_lv, lvs.data(), sizeof(LevelJoin) * lvs.size(), hipMemcpyHostToDevice, s));
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(d_bq, bq.data(), sizeof(BQuery) * nqs, hipMemcpyHostToDevice, s));
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(d_bq_, bq.data(), sizeof(BQuery) * nqs, hipMemcpyHostToDevice, s));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  prepared_ = true;
+  return CSM_OK;
+}
 
-  // 4. queues and pool
+csm_status BatchRun::launch(hipStream_t s) {
+  if (!prepared_) return set_error(ctx_, CSM_E_ARG, "batch launched before it was prepared");
+  StreamSwap swap(ctx_, s);
+  csm_ctx *ctx = ctx_;
+  const csm_graph *g = g_;
+  const csm_batch_opts &o = o_;
+  const uint32_t nqs = static_cast<uint32_t>(queries_.size()), words = words_;
   TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, o.block_kb ? o.block_kb : 128u,
                 o.ilp ? std::min(o.ilp, kIlp) : kIlp, o.max_out ? std::min(o.max_out, kMaxOut) : 1u};
   const uint64_t budget = o.item_budget ? o.item_budget : 2000000000ull;
@@ -1125,7 +1152,7 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   if (!pool_bytes) pool_bytes = std::min<uint64_t>(avail / 2, 32ull << 30);
   pool_bytes = std::min<uint64_t>(pool_bytes, avail * 3 / 4);
   const uint64_t num_blocks = std::max<uint64_t>(64, pool_bytes / (4 * block_words));
-  const uint32_t tasks_per_query = ceil_div(words, tp.root_words);
+  const uint32_t tasks_per_query = ceil_div(std::max<uint32_t>(words, 1), tp.root_words);
   const uint64_t nroot = static_cast<uint64_t>(nqs) * tasks_per_query;
   if (num_blocks + nroot >= kNil) return set_error(ctx, CSM_E_LIMIT, "too many pool blocks and root tasks");
   const uint32_t grid = ctx->num_cus * tp.wgs_per_cu;
@@ -1135,14 +1162,14 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   uint32_t *dq_buf = static_cast<uint32_t *>(get(4ull * kDequeCap * grid));
   FreeSlot *free_ring = static_cast<FreeSlot *>(get(sizeof(FreeSlot) * num_blocks));
   unsigned long long *ctr = static_cast<unsigned long long *>(get(8ull * 2));
-  BCtl *d_ctl = static_cast<BCtl *>(get(sizeof(BCtl)));
-  if (!pool || !recs || !dq || !dq_buf || !free_ring || !ctr || !d_ctl)
+  d_ctl_ = get(sizeof(BCtl));
+  if (!pool || !recs || !dq || !dq_buf || !free_ring || !ctr || !d_ctl_)
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (batch pool)");
   BArgs a{};
   a.probe = AdjProbe{g->off, g->adj, g->hub_id, g->hub_bits, words};
-  a.qs = d_bq;
-  a.st = d_st;
-  a.ctl = d_ctl;
+  a.qs = static_cast<const BQuery *>(d_bq_);
+  a.st = static_cast<BState *>(d_st_);
+  a.ctl = static_cast<BCtl *>(d_ctl_);
   a.recs = recs;
   a.dq = dq;
   a.dq_buf = dq_buf;
@@ -1163,69 +1190,78 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   a.max_out = std::min(tp.max_out, kMaxOut);
   const double limit_s = o.time_limit_ms ? o.time_limit_ms * 1e-3 : 120.0;
   a.deadline_ticks = static_cast<uint64_t>(limit_s * 1e8);  // 100 MHz real-time clock
-  const size_t lds_rows = 4ull * kRowBatch * std::max<uint32_t>(1, max_nq - 1);
+  grid_ = grid;
+  num_blocks_ = num_blocks;
+  const size_t lds_rows = 4ull * kRowBatch * std::max<uint32_t>(1, max_nq_ - 1);
   auto launch_join = [&]() {
     batch_init_kernel<<<grid_for(std::max<uint64_t>(num_blocks, grid)), 256, 0, s>>>(a, grid);
     if (tp.wgs_per_cu >= 4) batch_join_kernel<4><<<grid, kThreads, lds_rows, s>>>(a);
     else batch_join_kernel<2><<<grid, kThreads, lds_rows, s>>>(a);
   };
-  if (o.use_graph) {
-    hipGraph_t graph;
-    hipGraphExec_t exec;
+  join_kt_.reset(new KernelTimer(ctx, kstat_.data()));
+  KernelTimer &kt = *join_kt_;
+  if (o.use_graph) {  // the graph lives until finish() has waited for its launch
     CSM_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     launch_join();
-    CSM_HIP_TRY(ctx, hipStreamEndCapture(s, &graph));
-    CSM_HIP_TRY(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    CSM_HIP_TRY(ctx, hipStreamEndCapture(s, &graph_));
+    CSM_HIP_TRY(ctx, hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
     kt.begin(CSM_K_BATCH);
-    CSM_HIP_TRY(ctx, hipGraphLaunch(exec, s));
+    CSM_HIP_TRY(ctx, hipGraphLaunch(graph_exec_, s));
     kt.end(CSM_K_BATCH, 0);
-    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
   } else {
     kt.begin(CSM_K_BATCH);
     launch_join();
     kt.end(CSM_K_BATCH, 0);
   }
+  join_stream_ = s;
+  launched_ = true;
+  return CSM_OK;
+}
+
+csm_status BatchRun::finish(csm_result *results) {
+  csm_ctx *ctx = ctx_;
+  const uint32_t nqs = static_cast<uint32_t>(queries_.size());
+  const uint64_t cap = o_.cap ? o_.cap : ~0ull;
+  if (!launched_) return set_error(ctx, CSM_E_ARG, "batch finished before it was launched");
+  hipStream_t s = join_stream_;
   std::vector<BState> hst(nqs);
   BCtl hctl{};
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(hst.data(), d_st, sizeof(BState) * nqs, hipMemcpyDeviceToHost, s));
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(&hctl, d_ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s));
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(hst.data(), d_st_, sizeof(BState) * nqs, hipMemcpyDeviceToHost, s));
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(&hctl, d_ctl_, sizeof(BCtl), hipMemcpyDeviceToHost, s));
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-  kt.flush();
+  join_kt_->flush();
+  release_graph();
   // algorithmic bytes of the persistent join: rows read, per-row anchor list
   // lookups, list entries, rows written
-  const uint64_t join_bytes = hctl.rows_in * (4ull * 8 + 32) + hctl.items * 4 + hctl.rows_out * 4ull * 9;
-  kstat[CSM_K_BATCH].bytes = join_bytes;
+  kstat_[CSM_K_BATCH].bytes = hctl.rows_in * (4ull * 8 + 32) + hctl.items * 4 + hctl.rows_out * 4ull * 9;
   const char *tr = std::getenv("CSM_TRACE");
   const bool trace = tr && *tr == '1';
-  if (trace)
+  const uint32_t grid = grid_;
+  if (trace) {
     std::fprintf(stderr,
                  "csm batch: queries %u grid %u blocks %llu tasks %llu items %llu rows_in %llu rows_out %llu "
                  "steals %llu abort %u overflow %u fails %llu; per workgroup: busy %.1f ms idle %.1f ms sync %.1f ms\n",
-                 nqs, grid, static_cast<unsigned long long>(num_blocks), hctl.tasks, hctl.items, hctl.rows_in,
+                 nqs, grid, static_cast<unsigned long long>(num_blocks_), hctl.tasks, hctl.items, hctl.rows_in,
                  hctl.rows_out, hctl.steals, hctl.abort, hctl.overflow, hctl.fails, hctl.busy_ticks / 1e5 / grid,
                  hctl.idle_ticks / 1e5 / grid, hctl.sync_ticks / 1e5 / grid);
-  if (trace)
-    std::fprintf(stderr, "csm batch: per workgroup: staging %.1f ms, count tiles %.1f ms, extension tiles %.1f ms, task ends %.1f ms\n",
+    std::fprintf(stderr,
+                 "csm batch: per workgroup: staging %.1f ms, count tiles %.1f ms, extension tiles %.1f ms, task ends %.1f ms\n",
                  hctl.phase_ticks[0] / 1e5 / grid, hctl.phase_ticks[1] / 1e5 / grid, hctl.phase_ticks[2] / 1e5 / grid,
                  hctl.phase_ticks[3] / 1e5 / grid);
-  const double ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
   for (uint32_t i = 0; i < nqs; ++i) {
     csm_result &r = results[i];
     std::memset(&r, 0, sizeof(r));
-    const uint64_t total = bq[i].active ? hst[i].count : host_count[i];
+    const uint64_t total = active_[i] ? hst[i].count : host_count_[i];
     r.count = std::min(total, cap);
     r.capped = total > cap;
     r.status = CSM_OK;
-    r.ms_filter = ms_filter;
-    r.ms_total = ms_total;
-    r.ms_join = ms_total - ms_filter;
+    r.ms_filter = ms_filter_;
     r.items = hst[i].items;
     r.partial_rows = hst[i].rows;
-    r.path = bq[i].active ? CSM_PATH_BATCH : CSM_PATH_HOST;
+    r.path = active_[i] ? CSM_PATH_BATCH : CSM_PATH_HOST;
     if (i == 0) {
-      for (int c = 0; c < CSM_K_NUM; ++c) r.kernels[c] = kstat[c];
+      for (int c = 0; c < CSM_K_NUM; ++c) r.kernels[c] = kstat_[c];
       r.kernels[CSM_K_BATCH].launches = 1;
     }
   }
@@ -1233,14 +1269,14 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
   // are finished by the plain path (one at a time, exact)
   uint32_t fallbacks = 0;
   for (uint32_t i = 0; i < nqs; ++i) {
-    if (!bq[i].active || hst[i].done == kCapped) continue;
+    if (!active_[i] || hst[i].done == kCapped) continue;
     if (hst[i].done == kAbandoned || hctl.abort) {
       csm_match_opts mo{};
-      mo.cap = o.cap;
-      mo.refine = o.refine;
+      mo.cap = o_.cap;
+      mo.refine = o_.refine;
       mo.reuse_filter = 1;
       csm_result r{};
-      csm_status st = run_join(ctx, g, queries[i], mo, r);
+      csm_status st = run_join(ctx, g_, queries_[i], mo, r);
       if (st != CSM_OK) return st;
       results[i].count = r.count;
       results[i].capped = r.capped;
@@ -1250,13 +1286,23 @@ csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_que
       ++fallbacks;
     }
   }
-  const double ms_all = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const double ms_all = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
   for (uint32_t i = 0; i < nqs; ++i) {
     results[i].ms_total = ms_all;
-    results[i].ms_join = ms_all - ms_filter;
+    results[i].ms_join = ms_all - ms_filter_;
   }
   if (trace) std::fprintf(stderr, "csm batch: %u queries finished by the plain path, %.1f ms in all\n", fallbacks, ms_all);
   return CSM_OK;
+}
+
+csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_query *> &queries,
+                     const csm_batch_opts &o, csm_result *results) {
+  if (queries.empty()) return CSM_OK;
+  BatchRun run(ctx, g, queries, o);
+  csm_status st = run.prepare(ctx->stream);
+  if (st == CSM_OK) st = run.launch(ctx->stream);
+  if (st == CSM_OK) st = run.finish(results);
+  return st;
 }
 
 }  // namespace csm

@@ -47,7 +47,8 @@ def test_batch_small_random_vs_oracle(ctx):
                 res = csm.batch_match(g, qs, cap=cap, use_graph=use_graph, time_limit_ms=20000)
                 assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in res), [x.path for x in res]
                 got = [(x.count, bool(x.capped)) for x in res]
-                assert got == want, (round_, cap, use_graph)
+                bad = [(i, queries[i], got[i], want[i]) for i in range(len(got)) if got[i] != want[i]]
+                assert not bad, (round_, cap, use_graph, bad[:3])
         for q in qs:
             q.free()
         g.free()

@@ -155,3 +155,29 @@ def test_cli_argument_errors():
     r = run("-h")
     assert r.returncode == 0 and "-q" in r.stdout
     assert run("-q", qp, "-d", "/nonexistent/g.in").returncode == 1
+
+
+def test_abi_struct_layouts_match_ctypes(tmp_path):
+    """Every struct the C ABI passes by pointer has the same size (and the
+    later fields the same offsets) in include/csm.h and in the ctypes mirror."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not found")
+    src = tmp_path / "sizes.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "csm.h"\n'
+        'int main(void) {\n'
+        ' printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(csm_result), offsetof(csm_result, path),\n'
+        '  offsetof(csm_result, kernels), sizeof(csm_batch_opts), offsetof(csm_batch_opts, item_budget),\n'
+        '  sizeof(csm_serve_opts), sizeof(csm_match_opts), sizeof(csm_query_plan), sizeof(csm_kernel_stat),\n'
+        '  sizeof(csm_ctx_info));\n return 0;\n}\n')
+    exe = tmp_path / "sizes"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(csm.Result), csm.Result.path.offset, csm.Result.kernels.offset, C.sizeof(csm.BatchOpts),
+            csm.BatchOpts.item_budget.offset, C.sizeof(csm.ServeOpts), C.sizeof(csm.MatchOpts),
+            C.sizeof(csm.QueryPlan), C.sizeof(csm.KernelStat), C.sizeof(csm.CtxInfo)]
+    assert got == want
