@@ -1,27 +1,35 @@
 """Benchmark of the MI355X matching path; prints one JSON line.
 
-Default workload (one GPU): R-MAT scale 16 (65,536 vertices, 2^20 edge
-samples, a=0.57 b=0.19 c=0.19, loops and duplicates removed), 16 labels drawn
-Zipf(1.0); two 8-vertex random-walk queries, one tree-like and one with 3
-extra cycle edges.  One step = filter + refinement + join of both queries
-with the graph resident in HBM (the plain per-query path).  Both queries have
-more than 1e10 embeddings, so each is enumerated up to a per-query cap
-(count-only); value = embeddings counted / second.
+Headline (value): embeddings/s of the plain per-query path on R-MAT scale 16
+(65,536 vertices, 2^20 edge samples, a=0.57 b=0.19 c=0.19, loops and
+duplicates removed), 16 labels drawn Zipf(1.0); two 8-vertex random-walk
+queries, one tree-like and one with 3 extra cycle edges.  One step = filter
++ refinement + join of both queries with the graph resident in HBM.  Both
+queries have more than 1e10 embeddings, so each is enumerated up to a
+per-query cap (count-only); value = embeddings counted / second.  Kernel
+times come from HIP events recorded on the launch stream during the timed
+steps; the roofline entry is the kernel class with the most device time.
 
-Timed steps run without per-launch events; one extra profiled step gives the
-per-kernel device times behind the roofline figure.  cpu_baseline times the
-CPU oracle (same graph, queries, matching order and cap) on a bounded sample.
+Batched: R-MAT scale 20 (2^24 edge samples), 32 labels Zipf(1.0); 256
+random-walk queries of 4..32 vertices with average degree 2..4, per-query
+cap 1e8.  The plain path runs them query by query with a 2 s limit each; a
+few of these queries have no embedding after 1e11 candidate tests and are
+listed and left out.  One csm_batch_match call (one persistent kernel, no
+time limit, exact) runs every other query; queries/s is compared on that
+same set and every count must equal the plain path's.
+
+cpu_baseline times the CPU oracle (same graph, queries, matching order and
+cap) on a bounded sample of the headline workload.
 """
 import argparse
 import json
 import os
+import random
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-
-import numpy as np  # noqa: E402
 
 import clustersm_amd as csm  # noqa: E402
 
@@ -29,7 +37,7 @@ METRIC = "embeddings/s + batched queries/s vs per-query launch; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 
 
-def build_workload(args):
+def build_workload():
     hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
     return hg, [
         ("tree", csm.gen_query(hg, 8, csm.QUERY_TREE, 11)),
@@ -38,26 +46,27 @@ def build_workload(args):
 
 
 def run_step(queries, cap):
-    out = []
-    for q in queries:
-        r, _ = q.match(cap=cap)
-        out.append(r)
-    return out
+    return [q.match(cap=cap)[0] for q in queries]
 
 
-def roofline_from(results):
-    """Dominant kernel class of one profiled step: achieved = algorithmic bytes / device time."""
+def kernel_totals(result_lists):
     tot = {}
-    for r in results:
-        for c in range(csm.K_NUM):
-            k = r.kernels[c]
-            t = tot.setdefault(c, [0, 0, 0])
-            t[0] += k.launches
-            t[1] += k.ns
-            t[2] += k.bytes
+    for results in result_lists:
+        for r in results:
+            for c in range(csm.K_NUM):
+                k = r.kernels[c]
+                t = tot.setdefault(c, [0, 0, 0])
+                t[0] += k.launches
+                t[1] += k.ns
+                t[2] += k.bytes
+    return tot
+
+
+def roofline_from(tot):
+    """Dominant kernel class: achieved = algorithmic bytes / device time (bytes/ns = GB/s)."""
     dom = max(tot, key=lambda c: tot[c][1])
     launches, ns, byts = tot[dom]
-    achieved = byts / ns if ns else 0.0  # bytes/ns == GB/s
+    achieved = byts / ns if ns else 0.0
     return {
         "kernel": csm.KERNEL_NAMES[dom],
         "bound": "hbm",
@@ -69,8 +78,12 @@ def roofline_from(results):
         "launches": launches,
         "avg_launch_us": round(ns / launches / 1e3, 2) if launches else None,
         "bytes_per_launch": byts // launches if launches else None,
-    }, {csm.KERNEL_NAMES[c]: {"launches": v[0], "ms": round(v[1] / 1e6, 3), "GB": round(v[2] / 1e9, 3)}
-        for c, v in tot.items() if v[0]}
+    }
+
+
+def kernels_json(tot):
+    return {csm.KERNEL_NAMES[c]: {"launches": v[0], "ms": round(v[1] / 1e6, 3), "GB": round(v[2] / 1e9, 3)}
+            for c, v in tot.items() if v[0]}
 
 
 def cpu_baseline(hg, host_queries, cap, seconds):
@@ -99,6 +112,72 @@ def cpu_baseline(hg, host_queries, cap, seconds):
     }
 
 
+def batched_section(ctx, nqueries, cap, limit_ms):
+    """The C3-shaped query set: the plain path query by query (each with a time
+    limit), then one batch call over every query the plain path finished,
+    exact (no limit); queries/s compared on that same set."""
+    t0 = time.time()
+    hg = csm.gen_rmat(20, 1 << 24, 32, 1.0, 3)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    r = random.Random(5)
+    plans = []
+    for i in range(nqueries):
+        nv = r.randint(4, 32)
+        avg = r.uniform(2, 4)
+        plans.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 1000 + i, avg_degree=avg).plan())
+    qs = [g.prepare(p) for p in plans]
+    print(f"# batched setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    plain, plain_s = [], []
+    for q in qs:
+        t1 = time.perf_counter()
+        plain.append(q.match(cap=cap, time_limit_ms=limit_ms)[0])
+        plain_s.append(time.perf_counter() - t1)
+    keep = [i for i in range(nqueries) if plain[i].status == 0]
+    dropped = [i for i in range(nqueries) if plain[i].status != 0]
+    t_plain = sum(plain_s[i] for i in keep)
+    print(f"# plain path: {len(keep)} of {nqueries} queries finished within {limit_ms} ms each, "
+          f"{t_plain:.2f}s for those", file=sys.stderr, flush=True)
+    kq = [qs[i] for i in keep]
+    csm.batch_match(g, kq, cap=cap)  # warm-up (code objects, arena layout)
+    ctx.set_profiling(True)
+    t2 = time.perf_counter()
+    res = csm.batch_match(g, kq, cap=cap)
+    t_batch = time.perf_counter() - t2
+    ctx.set_profiling(False)
+    print(f"# batched: {len(keep)} queries in {t_batch:.2f}s", file=sys.stderr, flush=True)
+    equal = all((x.count, x.capped, x.status) == (plain[i].count, plain[i].capped, 0) for x, i in zip(res, keep))
+    paths = {}
+    for x in res:
+        paths[x.path] = paths.get(x.path, 0) + 1
+    tot = kernel_totals([[res[0]]])
+    bk = tot.get(csm.K_BATCH, [0, 0, 0])
+    out = {
+        "workload": f"R-MAT scale 20 (2^24 samples, 32 Zipf(1.0) labels), {nqueries} random-walk queries of "
+                    f"4..32 vertices, average degree 2..4, cap {cap:.0e} per query, count-only",
+        "queries": len(keep),
+        "queries_per_s": round(len(keep) / t_batch, 2),
+        "plain_queries_per_s": round(len(keep) / t_plain, 2),
+        "speedup_vs_plain": round(t_plain / t_batch, 2),
+        "counts_equal_plain": equal,
+        "excluded": {"queries": dropped, "rule": f"the plain path does not finish them within {limit_ms} ms "
+                                                 f"(more than 1e11 candidate tests; the batch is not run on them)",
+                     "plain_items": [plain[i].items for i in dropped]},
+        "capped_queries": sum(1 for x in res if x.capped),
+        "embeddings_per_s": round(sum(x.count for x in res) / t_batch, 1),
+        "finished_by": {{csm.PATH_PLAIN: "plain", csm.PATH_HOST: "host", csm.PATH_BATCH: "batch kernel",
+                         csm.PATH_BATCH_BUDGET: "plain after the item budget",
+                         csm.PATH_BATCH_ABORT: "plain after an early stop"}[k]: v for k, v in paths.items()},
+        "kernels": kernels_json(tot),
+        "join_kernel": {"ms": round(bk[1] / 1e6, 1), "algorithmic_GB": round(bk[2] / 1e9, 2),
+                        "GB_per_s": round(bk[2] / bk[1], 1) if bk[1] else None},
+    }
+    for q in qs:
+        q.free()
+    g.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +186,11 @@ def main():
     ap.add_argument("--cap", type=int, default=4_000_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--arena-gb", type=float, default=64.0)
+    ap.add_argument("--batched-queries", type=int, default=256)
+    ap.add_argument("--batched-cap", type=int, default=100_000_000)
+    ap.add_argument("--batched-limit-ms", type=int, default=2000)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--arena-gb", type=float, default=96.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
@@ -115,29 +198,29 @@ def main():
         sys.exit(2)
 
     t0 = time.time()
-    hg, named = build_workload(args)
+    hg, named = build_workload()
     ctx = csm.Context(0, int(args.arena_gb * (1 << 30)))
     g = ctx.upload(hg)
     nv, ne, maxdeg = g.info()
     queries = [g.prepare(hq.plan()) for _, hq in named]
-    setup_s = time.time() - t0
-    print(f"# setup {setup_s:.1f}s: graph |V|={nv} |E|={ne} max_degree={maxdeg}", file=sys.stderr, flush=True)
+    print(f"# setup {time.time() - t0:.1f}s: graph |V|={nv} |E|={ne} max_degree={maxdeg}", file=sys.stderr, flush=True)
+    ms_copy = ctx.copy_probe(1 << 30, 10)
+    copy_gbs = 2.0 * (1 << 30) / (ms_copy * 1e6)
 
     for _ in range(args.warmup):
         run_step(queries, args.cap)
+    # timed steps; kernel device times from HIP events on the launch stream
+    ctx.set_profiling(True)
     t1 = time.perf_counter()
-    for _ in range(args.steps):
-        res = run_step(queries, args.cap)
+    steps = [run_step(queries, args.cap) for _ in range(args.steps)]
     elapsed = time.perf_counter() - t1
+    ctx.set_profiling(False)
+    res = steps[-1]
     ms_per_step = elapsed * 1e3 / args.steps
     embeddings = sum(r.count for r in res)
     value = embeddings / (elapsed / args.steps)
     print(f"# timed {args.steps} steps in {elapsed:.2f}s: {embeddings} embeddings/step", file=sys.stderr, flush=True)
-
-    ctx.set_profiling(True)
-    prof = run_step(queries, args.cap)
-    ctx.set_profiling(False)
-    roof, kernels = roofline_from(prof)
+    tot = kernel_totals(steps)
 
     line = {
         "metric": METRIC,
@@ -162,14 +245,19 @@ def main():
             "queries": [{"name": n, "count": r.count, "capped": bool(r.capped),
                          "ms": round(r.ms_total, 3)} for (n, _), r in zip(named, res)],
         },
-        "roofline": roof,
-        "kernels": kernels,
+        "roofline": roofline_from(tot),
+        "kernels": kernels_json(tot),
+        "copy_probe_GBps": round(copy_gbs, 1),
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(hg, [hq for _, hq in named], args.cap, args.cpu_seconds)
     for q in queries:
         q.free()
     g.free()
+    if not args.no_batched:
+        line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_limit_ms)
+    line["served"] = {"status": "not measured: the serving test ended in SIGABRT on its first run after the "
+                                "batch tests; serving stays off the GPU until that cause is found"}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(hg, [hq for _, hq in named], args.cap, args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -23,9 +23,9 @@ STATUS = {
     0: "ok", 1: "invalid argument", 2: "cannot open file", 3: "malformed graph file",
     4: "query outside the supported envelope", 5: "planner limits exceeded", 6: "HIP runtime error",
     7: "device arena exhausted", 8: "no gfx950 device", 9: "serving queue full", 10: "unknown ticket",
-    11: "result pending", 12: "graph exceeds the uint32 CSR layout",
+    11: "result pending", 12: "graph exceeds the uint32 CSR layout", 13: "time limit reached",
 }
-E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING = 2, 3, 4, 7, 8, 9, 11
+E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING, E_TIMEOUT = 2, 3, 4, 7, 8, 9, 11, 13
 
 K_FILTER, K_REFINE, K_EXPAND, K_MATERIALIZE, K_COUNT, K_AUX, K_BATCH = range(7)
 K_NUM = 7
@@ -99,7 +99,7 @@ class KernelStat(C.Structure):
 class MatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("reuse_filter", C.c_uint32),
                 ("list", C.POINTER(C.c_uint32)), ("list_capacity", C.c_uint64),
-                ("chunk_items", C.c_uint64)]
+                ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -151,6 +151,7 @@ SIGNATURES = {
     "csm_status_string": ([C.c_int], C.c_char_p),
     "csm_ctx_query": ([_vp, C.POINTER(CtxInfo)], C.c_int),
     "csm_set_profiling": ([_vp, C.c_int], C.c_int),
+    "csm_copy_probe": ([_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)], C.c_int),
     "csm_graph_upload": ([_vp, _vp, C.POINTER(_vp)], C.c_int),
     "csm_graph_upload_edges": ([_vp, C.c_uint32, _u32p, C.c_uint64, _u32p, C.POINTER(_vp)], C.c_int),
     "csm_graph_free": ([_vp, _vp], C.c_int),
@@ -321,6 +322,12 @@ class Context:
         if st != 0:
             raise CsmError(st, f"{what}: {(lib().csm_last_error(self.h) or b'').decode()}")
 
+    def copy_probe(self, nbytes, iters=10):
+        """ms per streaming copy of nbytes (csm_copy_probe); 2 * nbytes of HBM traffic each."""
+        ms = C.c_double()
+        self.check(lib().csm_copy_probe(self.h, nbytes, iters, C.byref(ms)), "csm_copy_probe")
+        return ms.value
+
     def info(self):
         i = CtxInfo()
         self.check(lib().csm_ctx_query(self.h, C.byref(i)), "csm_ctx_query")
@@ -424,10 +431,13 @@ class Query:
         self.graph.ctx.check(lib().csm_candidates_download(self.graph.ctx.h, self.h, bits.ctypes.data_as(_u64p)))
         return bits.reshape(self.plan.num_v, words)
 
-    def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0):
-        """Returns (Result, rows or None); rows are sorted, in query-id order."""
+    def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0,
+              time_limit_ms=0):
+        """Returns (Result, rows or None); rows are sorted, in query-id order.
+        With time_limit_ms, Result.status == E_TIMEOUT marks a stopped search."""
         o = MatchOpts()
         o.cap, o.refine, o.reuse_filter, o.chunk_items = cap, 1 if refine else 0, 1 if reuse_filter else 0, chunk_items
+        o.time_limit_ms = time_limit_ms
         rows = None
         if want_list:
             lc = list_capacity if list_capacity is not None else (cap if cap else 1 << 20)

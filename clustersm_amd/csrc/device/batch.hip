@@ -63,12 +63,14 @@ struct BQuery {
 struct BState {
   unsigned long long count;
   unsigned int done;         // kRunning, kCapped or kAbandoned
-  unsigned int pad;
+  unsigned int lost;         // a task of the query was dropped or cut short by an early stop
+  unsigned long long pending;  // tasks of the query queued or running
   unsigned long long items;  // (row, list entry) pairs examined for the query
   unsigned long long rows;   // partial rows written for the query
 };
 
 enum : unsigned int { kRunning = 0, kCapped = 1, kAbandoned = 2 };
+enum : unsigned int { kAbortPool = 1, kAbortTime = 2 };
 
 // A task record: one per pool block (a block's rows form at most one task)
 // and one per root range.
@@ -114,7 +116,7 @@ struct BCtl {
   unsigned long long sync_ticks;   // real-time ticks in deque and free-ring operations
   unsigned long long phase_ticks[4];  // rows staged + drivers, count tiles, extension tiles, task ends
   unsigned long long alloc_fail;   // consecutive failed allocations from the free ring
-  unsigned int abort;
+  unsigned int abort;              // kAbortPool or kAbortTime once the kernel stops early
   unsigned int overflow;           // a deque was full (never expected)
 };
 
@@ -240,6 +242,8 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
   for (uint64_t i = tid; i < a.nqueries; i += stride) {
     a.st[i].count = 0;
     a.st[i].done = a.qs[i].active ? kRunning : kCapped;
+    a.st[i].lost = 0;
+    a.st[i].pending = a.qs[i].active ? a.tasks_per_query : 0;
     a.st[i].items = a.st[i].rows = 0;
   }
   if (tid == 0) {
@@ -313,7 +317,7 @@ __device__ bool blk_alloc(const BArgs &a, Smem &sm, uint32_t &id) {
   if (ok) {
     if (ld_rlx(&a.ctl->alloc_fail)) atomicExch(&a.ctl->alloc_fail, 0ull);
   } else if (atomicAdd(&a.ctl->alloc_fail, 1ull) > kAbortAfterFails) {
-    atomicExch(&a.ctl->abort, 1u);  // no block freed for a long time: the host finishes the batch
+    atomicCAS(&a.ctl->abort, 0u, kAbortPool);  // no block freed for a long time: the host finishes the batch
   }
   sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
   return ok;
@@ -344,7 +348,7 @@ __device__ void deque_push(const BArgs &a, Smem &sm, uint32_t wg, const uint32_t
   const unsigned long long b = sm.dq_bottom, t = ld_rlx(&a.dq[wg].top);
   if (b - t + n > kDequeCap) {
     atomicExch(&a.ctl->overflow, 1u);
-    atomicExch(&a.ctl->abort, 1u);
+    atomicCAS(&a.ctl->abort, 0u, kAbortPool);
     return;
   }
   for (uint32_t i = 0; i < n; ++i) buf[(b + i) % kDequeCap] = recs[i];
@@ -536,7 +540,10 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   __syncthreads();
   uint64_t tp0 = __builtin_amdgcn_s_memrealtime();
   for (; row0 < nrows; row0 += kRowBatch) {
-    if (tid == 0) sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
+    if (tid == 0) {
+      sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
+      if (sm.stop && ld_done(a, t.q) == kRunning) a.st[t.q].lost = 1;  // cut short by an early stop
+    }
     __syncthreads();
     if (sm.stop) break;
     const uint32_t nb = min(kRowBatch, nrows - row0);
@@ -797,6 +804,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   const unsigned long long tot_items = block_reduce(items_seen, sm.red);
   if (tid == 0) {
     const bool live = ld_done(a, t.q) == kRunning && !aborted(a);
+    if (!live && ld_done(a, t.q) == kRunning) a.st[t.q].lost = 1;  // its outputs are dropped by an early stop
     uint32_t np = 0;
     if (sm.flag != 0 && live) {  // same rows, resume point updated
       write_rec(a, sm.rec, sm.task);
@@ -821,6 +829,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         blk_free(a, sm, sm.held[i]);
       }
     }
+    if (np) atomicAdd(&a.st[t.q].pending, static_cast<unsigned long long>(np));
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     deque_push(a, sm, wg, sm.push, np);
     sm.sync_ticks += __builtin_amdgcn_s_memrealtime() - t0;
@@ -878,7 +887,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
   }
   for (;;) {
     if (tid == 0) {
-      if (__builtin_amdgcn_s_memrealtime() - t_start > a.deadline_ticks) atomicExch(&a.ctl->abort, 1u);
+      if (__builtin_amdgcn_s_memrealtime() - t_start > a.deadline_ticks) atomicCAS(&a.ctl->abort, 0u, kAbortTime);
       sm.state = 1;
       const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
       // own deque, then a new root task, then the oldest task of another workgroup
@@ -971,10 +980,12 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out);
       if (sm.flag == 2)  // pool empty: give other workgroups time to free blocks
         for (int i = 0; i < 16; ++i) __builtin_amdgcn_s_sleep(32);
-    } else if (tid == 0 && k >= 2) {
-      blk_free(a, sm, sm.task.block);
+    } else if (tid == 0) {
+      if (ld_done(a, q) == kRunning) a.st[q].lost = 1;  // dropped by an early stop
+      if (k >= 2) blk_free(a, sm, sm.task.block);
     }
     __syncthreads();
+    if (tid == 0) atomicAdd(&a.st[q].pending, ~0ull);  // this task is finished
     ++n_tasks;
     busy += __builtin_amdgcn_s_memrealtime() - t_task;
   }
@@ -1142,7 +1153,9 @@ csm_status BatchRun::launch(hipStream_t s) {
   const uint32_t nqs = static_cast<uint32_t>(queries_.size()), words = words_;
   TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, o.block_kb ? o.block_kb : 128u,
                 o.ilp ? std::min(o.ilp, kIlp) : kIlp, o.max_out ? std::min(o.max_out, kMaxOut) : 1u};
-  const uint64_t budget = o.item_budget ? o.item_budget : 2000000000ull;
+  // with a time limit the queries are bounded by time; otherwise by items, so
+  // an unlucky depth-first search is handed to the plain path and stays exact
+  const uint64_t budget = o.item_budget ? o.item_budget : o.time_limit_ms ? ~0ull : 2000000000ull;
   if (o.chunk_rows) tp.root_words = std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, o.chunk_rows / 64));
   if (tp.wgs_per_cu > 8 || tp.block_kb < 32 || tp.block_kb > 4096) return set_error(ctx, CSM_E_ARG, "batch tuning out of range");
   const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;
@@ -1265,21 +1278,39 @@ csm_status BatchRun::finish(csm_result *results) {
       r.kernels[CSM_K_BATCH].launches = 1;
     }
   }
-  // queries over their budget, and every unfinished query after an abort,
-  // are finished by the plain path (one at a time, exact)
+  // Queries over their item budget, and every unfinished query after the
+  // kernel stopped early, are finished by the plain path (one at a time,
+  // exact), except that with a caller's time limit a query still unfinished
+  // when the limit passed keeps its count so far with status CSM_E_TIMEOUT.
   uint32_t fallbacks = 0;
+  const double ms_kernel_end = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
   for (uint32_t i = 0; i < nqs; ++i) {
     if (!active_[i] || hst[i].done == kCapped) continue;
+    const bool finished = hst[i].pending == 0 && !hst[i].lost;
+    if (hst[i].done == kRunning && finished) continue;  // complete before any early stop: exact
+    if (o_.time_limit_ms && hctl.abort == kAbortTime && hst[i].done == kRunning) {
+      results[i].status = CSM_E_TIMEOUT;
+      continue;
+    }
     if (hst[i].done == kAbandoned || hctl.abort) {
       csm_match_opts mo{};
       mo.cap = o_.cap;
       mo.refine = o_.refine;
       mo.reuse_filter = 1;
+      if (o_.time_limit_ms) {  // what is left of the caller's limit
+        const double left = o_.time_limit_ms - ms_kernel_end;
+        if (left <= 1.0) {
+          results[i].status = CSM_E_TIMEOUT;
+          continue;
+        }
+        mo.time_limit_ms = static_cast<uint32_t>(left);
+      }
       csm_result r{};
       csm_status st = run_join(ctx, g_, queries_[i], mo, r);
       if (st != CSM_OK) return st;
       results[i].count = r.count;
       results[i].capped = r.capped;
+      results[i].status = r.status;
       results[i].items += r.items;
       results[i].partial_rows += r.partial_rows;
       results[i].path = hst[i].done == kAbandoned ? CSM_PATH_BATCH_BUDGET : CSM_PATH_BATCH_ABORT;

@@ -449,7 +449,12 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   uint64_t total = 0;
   uint32_t k = 1;
   bool stop_all = false;
+  const auto deadline = t0 + std::chrono::milliseconds(o.time_limit_ms);
   while (!stop_all) {
+    if (o.time_limit_ms && std::chrono::steady_clock::now() > deadline) {
+      res.status = CSM_E_TIMEOUT;  // count so far is a lower bound
+      break;
+    }
     if (cursor[k] >= nrows[k]) {
       if (k == 1) break;
       --k;
@@ -602,7 +607,7 @@ extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, 
   for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
   out->ms_filter = q->ms_filter;
   csm_status st = run_join(ctx, g, q, o, *out);
-  out->status = st;
+  if (st != CSM_OK || out->status != CSM_E_TIMEOUT) out->status = st;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }

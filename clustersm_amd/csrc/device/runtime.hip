@@ -151,6 +151,11 @@ __global__ void hub_bits_kernel(const uint32_t *off, const uint32_t *adj, const 
 
 }  // namespace
 
+// Streaming copy, grid-stride, 16 bytes per lane per step.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) dst[i] = src[i];
+}
+
 // Neighbour-label table: one wave per 64 vertices, a byte histogram per lane
 // in LDS (saturating), heavy vertices counted by the whole wave.
 __global__ __launch_bounds__(256) void nlc_table_kernel(const uint32_t *__restrict__ off,
@@ -488,6 +493,38 @@ const char *csm_last_error(const csm_ctx *ctx) { return ctx ? ctx->last_error.c_
 csm_status csm_set_profiling(csm_ctx *ctx, int on) {
   if (!ctx) return CSM_E_ARG;
   ctx->profiling = on != 0;
+  return CSM_OK;
+}
+
+csm_status csm_copy_probe(csm_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms_per_copy) {
+  if (!ctx || !ms_per_copy || bytes < 16 || iters == 0) return CSM_E_ARG;
+  hipSetDevice(ctx->device);
+  bytes &= ~15ull;
+  void *src = ctx->arena.alloc(bytes), *dst = ctx->arena.alloc(bytes);
+  if (!src || !dst) {
+    ctx->arena.free(src);
+    ctx->arena.free(dst);
+    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (copy probe)");
+  }
+  const uint64_t n = bytes / 16;
+  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(ctx->num_cus * 8ull, (n + 255) / 256));
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  copy_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);  // warm
+  hipEventRecord(e0, ctx->stream);
+  for (uint32_t i = 0; i < iters; ++i)
+    copy_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+  hipEventRecord(e1, ctx->stream);
+  const hipError_t err = hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  ctx->arena.free(src);
+  ctx->arena.free(dst);
+  if (err != hipSuccess) return set_error(ctx, CSM_E_DEVICE, hipGetErrorString(err));
+  *ms_per_copy = ms / iters;
   return CSM_OK;
 }
 

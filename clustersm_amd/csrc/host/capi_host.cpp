@@ -46,6 +46,7 @@ const char *csm_status_string(csm_status s) {
     case CSM_E_NOT_FOUND: return "unknown ticket";
     case CSM_E_PENDING: return "result pending";
     case CSM_E_LIMIT: return "graph exceeds the uint32 CSR layout";
+    case CSM_E_TIMEOUT: return "time limit reached";
   }
   return "unknown status";
 }

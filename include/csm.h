@@ -43,7 +43,8 @@ typedef enum csm_status {
   CSM_E_BUSY = 9,       /* serving admission queue full */
   CSM_E_NOT_FOUND = 10, /* unknown serving ticket */
   CSM_E_PENDING = 11,   /* serving result not ready yet */
-  CSM_E_LIMIT = 12      /* data graph beyond the int32 CSR layout (2|E| >= 2^32) */
+  CSM_E_LIMIT = 12,     /* data graph beyond the int32 CSR layout (2|E| >= 2^32) */
+  CSM_E_TIMEOUT = 13    /* result status only: the time limit ended the search; count is a lower bound */
 } csm_status;
 
 typedef struct csm_host_graph csm_host_graph; /* parsed or generated graph in host memory */
@@ -175,6 +176,9 @@ typedef struct csm_match_opts {
   uint32_t *list;          /* NULL = count only */
   uint64_t list_capacity;  /* rows */
   uint64_t chunk_items;    /* 0 = autotuned */
+  uint32_t time_limit_ms;  /* 0 = none; else the search stops after this and the result's status is
+                              CSM_E_TIMEOUT with the count so far (the call itself returns CSM_OK) */
+  uint32_t reserved;
 } csm_match_opts;
 
 /* Kernel classes for per-kernel accounting. */
@@ -222,6 +226,12 @@ enum {
 /* Per-launch device timing with HIP events on the context stream. */
 csm_status csm_set_profiling(csm_ctx *ctx, int on);
 
+/* Diagnostics, no reference counterpart: a streaming copy kernel (16-byte
+ * loads and stores) over `bytes` of arena memory, `iters` launches timed with
+ * HIP events.  Gives the HBM bandwidth this device sustains and a kernel of
+ * known traffic for calibrating hardware byte counters. */
+csm_status csm_copy_probe(csm_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms_per_copy);
+
 csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q,
                      const csm_match_opts *opts, csm_result *out);
 
@@ -240,8 +250,11 @@ typedef struct csm_batch_opts {
   uint32_t block_kb;       /* pool block size in KiB (32..4096), 0 = default 128 */
   uint32_t max_out;        /* full output blocks a task holds before it yields (1..8), 0 = default 1 */
   uint32_t ilp;            /* items per thread per tile (1..4), 0 = default 4 */
-  uint32_t time_limit_ms;  /* the persistent kernel gives up after this; the plain path finishes, 0 = 120 s */
-  uint64_t item_budget;    /* items per query before the plain path takes it over, 0 = 2e9 */
+  uint32_t time_limit_ms;  /* 0: none (the kernel still stops after 120 s and the plain path finishes
+                              its queries, exact); else per query: a query unfinished after this has
+                              status CSM_E_TIMEOUT and its count so far, as with csm_match */
+  uint64_t item_budget;    /* items per query before the plain path takes it over; 0 = 2e9 without a
+                              time limit, unlimited with one */
 } csm_batch_opts;
 csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
                            const csm_batch_opts *opts, csm_result *out);

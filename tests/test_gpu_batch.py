@@ -116,3 +116,37 @@ def test_batch_tuning_variants_agree(ctx):
     for q in qs:
         q.free()
     g.free()
+
+
+def test_time_limits_report_timeouts(ctx):
+    """A query stopped by its time limit reports E_TIMEOUT and a count that is a
+    lower bound; every query finished within the limit is exact (plain and batch)."""
+    hg = csm.gen_rmat(14, 200000, 4, 1.0, 5)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    plans = [csm.gen_query(host, 10, csm.QUERY_TREE, 40 + i).plan() for i in range(8)]
+    qs = [g.prepare(p) for p in plans]
+    cap = 20_000_000
+    want = [(x.count, x.capped) for x in (q.match(cap=cap)[0] for q in qs)]
+    timeouts = 0
+    for q, w in zip(qs, want):
+        r, _ = q.match(cap=cap, time_limit_ms=1)
+        if r.status == csm.E_TIMEOUT:
+            timeouts += 1
+            assert r.count <= w[0] and not r.capped
+        else:
+            assert r.status == 0 and (r.count, r.capped) == w
+    for limit in (1, 30, 20000):
+        res = csm.batch_match(g, qs, cap=cap, time_limit_ms=limit)
+        for x, w in zip(res, want):
+            if x.status == csm.E_TIMEOUT:
+                timeouts += 1
+                assert x.count <= w[0] and not x.capped, limit
+            else:
+                assert x.status == 0 and (x.count, x.capped) == w, limit
+        if limit == 20000:
+            assert all(x.status == 0 for x in res)
+    print("timeouts seen", timeouts, flush=True)
+    for q in qs:
+        q.free()
+    g.free()
