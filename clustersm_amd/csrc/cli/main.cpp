@@ -168,6 +168,7 @@ int main(int argc, char **argv) {
     csm_batch_opts bo{};
     bo.cap = cap;
     bo.refine = refine;
+    bo.use_graph = 1;
     std::vector<csm_result> rs(qs.size());
     st = csm_batch_match(ctx, g, qs.data(), static_cast<uint32_t>(qs.size()), &bo, rs.data());
     if (st != CSM_OK) die(st, "batch", csm_last_error(ctx));

@@ -5,7 +5,7 @@
  * cpu_baseline leg of bench.py.  The product never links it.
  *
  * Every function cites the reference file:line it restates.  How each part
- * is pinned (tests/test_oracle.py, tests/test_planner.py):
+ * is pinned (tests/test_oracle.py, tests/test_host.py):
  *   oc_query_read / oc_data_read  bundled example files; declared-degree sum
  *                                 = 2M; eid rule 2k / 2k+1 (io.cu:82-94)
  *   decycle                       the reference's own decycle.cpp compiled
