@@ -62,11 +62,25 @@ def kernel_totals(result_lists):
     return tot
 
 
+def measured_traffic():
+    """HBM bytes per launch by kernel class, from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated with a copy kernel)."""
+    out = {}
+    path = os.path.join(ROOT, "profiles", "count_kernel_rmat16_plain.md")
+    if os.path.exists(path):
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                cls, val = ln.split(":", 1)[1].split()
+                out[cls] = int(val)
+    return out
+
+
 def roofline_from(tot):
     """Dominant kernel class: achieved = algorithmic bytes / device time (bytes/ns = GB/s)."""
     dom = max(tot, key=lambda c: tot[c][1])
     launches, ns, byts = tot[dom]
     achieved = byts / ns if ns else 0.0
+    traffic = measured_traffic().get(csm.KERNEL_NAMES[dom])
     return {
         "kernel": csm.KERNEL_NAMES[dom],
         "bound": "hbm",
@@ -74,7 +88,7 @@ def roofline_from(tot):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
         "launches": launches,
         "avg_launch_us": round(ns / launches / 1e3, 2) if launches else None,
         "bytes_per_launch": byts // launches if launches else None,
