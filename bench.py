@@ -126,7 +126,7 @@ def cpu_baseline(hg, host_queries, cap, seconds):
     }
 
 
-def batched_section(ctx, nqueries, cap, limit_ms):
+def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0):
     """The C3-shaped query set: the plain path query by query (each with a time
     limit), then one batch call over every query the plain path finished,
     exact (no limit); queries/s compared on that same set."""
@@ -153,10 +153,17 @@ def batched_section(ctx, nqueries, cap, limit_ms):
     print(f"# plain path: {len(keep)} of {nqueries} queries finished within {limit_ms} ms each, "
           f"{t_plain:.2f}s for those", file=sys.stderr, flush=True)
     kq = [qs[i] for i in keep]
-    csm.batch_match(g, kq, cap=cap)  # warm-up (code objects, arena layout)
+    t3 = time.perf_counter()
+    tuned = csm.batch_autotune(g, kq, cap=cap, sample=64)  # untimed: a one-off per graph and query mix
+    if item_budget:
+        tuned.item_budget = item_budget
+    tune_s = time.perf_counter() - t3
+    print(f"# autotune {tune_s:.1f}s: wgs_per_cu={tuned.wgs_per_cu} max_out={tuned.max_out} "
+          f"block_kb={tuned.block_kb} ilp={tuned.ilp}", file=sys.stderr, flush=True)
+    csm.batch_match(g, kq, opts=tuned)  # warm-up (code objects, arena layout)
     ctx.set_profiling(True)
     t2 = time.perf_counter()
-    res = csm.batch_match(g, kq, cap=cap)
+    res = csm.batch_match(g, kq, opts=tuned)
     t_batch = time.perf_counter() - t2
     ctx.set_profiling(False)
     print(f"# batched: {len(keep)} queries in {t_batch:.2f}s", file=sys.stderr, flush=True)
@@ -174,6 +181,8 @@ def batched_section(ctx, nqueries, cap, limit_ms):
         "plain_queries_per_s": round(len(keep) / t_plain, 2),
         "speedup_vs_plain": round(t_plain / t_batch, 2),
         "counts_equal_plain": equal,
+        "autotuned": {"wgs_per_cu": tuned.wgs_per_cu, "max_out": tuned.max_out, "block_kb": tuned.block_kb,
+                      "ilp": tuned.ilp, "sample_queries": min(64, len(keep)), "tuning_s": round(tune_s, 1)},
         "excluded": {"queries": dropped, "rule": f"the plain path does not finish them within {limit_ms} ms "
                                                  f"(more than 1e11 candidate tests; the batch is not run on them)",
                      "plain_items": [plain[i].items for i in dropped]},
@@ -203,6 +212,7 @@ def main():
     ap.add_argument("--batched-queries", type=int, default=256)
     ap.add_argument("--batched-cap", type=int, default=100_000_000)
     ap.add_argument("--batched-limit-ms", type=int, default=2000)
+    ap.add_argument("--batched-item-budget", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--arena-gb", type=float, default=96.0)
     args = ap.parse_args()
@@ -267,7 +277,8 @@ def main():
         q.free()
     g.free()
     if not args.no_batched:
-        line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_limit_ms)
+        line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_limit_ms,
+                                          args.batched_item_budget)
     line["served"] = {"status": "not measured: the serving test ended in SIGABRT on its first run after the "
                                 "batch tests; serving stays off the GPU until that cause is found"}
     if not args.no_cpu:

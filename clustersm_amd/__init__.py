@@ -164,6 +164,8 @@ SIGNATURES = {
     "csm_candidates_download": ([_vp, _vp, _u64p], C.c_int),
     "csm_match": ([_vp, _vp, _vp, C.POINTER(MatchOpts), C.POINTER(Result)], C.c_int),
     "csm_batch_match": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.POINTER(Result)], C.c_int),
+    "csm_batch_autotune": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.c_uint32,
+                            C.POINTER(BatchOpts)], C.c_int),
     "csm_serve_start": ([_vp, _vp, C.POINTER(ServeOpts)], C.c_int),
     "csm_serve_submit": ([_vp, C.POINTER(QueryPlan), _u64p], C.c_int),
     "csm_serve_poll": ([_vp, C.c_uint64, C.c_int, C.POINTER(Result)], C.c_int),
@@ -452,12 +454,24 @@ class Query:
         return r, rows
 
 
-def batch_match(graph, queries, cap=0, refine=False, use_graph=True, chunk_rows=0, wgs_per_cu=0, pool_bytes=0,
-                block_kb=0, max_out=0, ilp=0, time_limit_ms=0, item_budget=0):
-    """All queries in one persistent kernel (csm_batch_match); counts only."""
+def batch_autotune(graph, queries, cap=0, refine=False, sample=32, time_limit_ms=0):
+    """Tuned BatchOpts for this graph and query mix (csm_batch_autotune)."""
     arr = (_vp * len(queries))(*[q.h for q in queries])
-    o = BatchOpts(cap, 1 if refine else 0, 1 if use_graph else 0, chunk_rows, wgs_per_cu, pool_bytes,
-                  block_kb, max_out, ilp, time_limit_ms, item_budget)
+    base = BatchOpts(cap, 1 if refine else 0, 1, 0, 0, 0, 0, 0, 0, time_limit_ms, 0)
+    out = BatchOpts()
+    graph.ctx.check(lib().csm_batch_autotune(graph.ctx.h, graph.h, arr, len(queries), C.byref(base), sample,
+                                             C.byref(out)), "csm_batch_autotune")
+    return out
+
+
+def batch_match(graph, queries, cap=0, refine=False, use_graph=True, chunk_rows=0, wgs_per_cu=0, pool_bytes=0,
+                block_kb=0, max_out=0, ilp=0, time_limit_ms=0, item_budget=0, opts=None):
+    """All queries in one persistent kernel (csm_batch_match); counts only.
+    opts: a BatchOpts (e.g. from batch_autotune) used as given."""
+    arr = (_vp * len(queries))(*[q.h for q in queries])
+    o = opts if opts is not None else BatchOpts(cap, 1 if refine else 0, 1 if use_graph else 0, chunk_rows,
+                                                wgs_per_cu, pool_bytes, block_kb, max_out, ilp, time_limit_ms,
+                                                item_budget)
     res = (Result * len(queries))()
     graph.ctx.check(lib().csm_batch_match(graph.ctx.h, graph.h, arr, len(queries), C.byref(o), res),
                     "csm_batch_match")

@@ -258,6 +258,13 @@ typedef struct csm_batch_opts {
 } csm_batch_opts;
 csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
                            const csm_batch_opts *opts, csm_result *out);
+/* Autotuner, no reference counterpart: times csm_batch_match on up to
+ * `sample` of the queries (evenly spaced; cap, refine and limits from `base`)
+ * for a small grid of workgroups per CU, output blocks per task, block size
+ * and items per thread, and writes the fastest setting into *out (the other
+ * fields copied from `base`).  Results of the trial runs are discarded. */
+csm_status csm_batch_autotune(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
+                              const csm_batch_opts *base, uint32_t sample, csm_batch_opts *out);
 
 /* ----------------------------------------------------------------- serving
  * No reference counterpart.  A request loop keeps the graph resident, admits

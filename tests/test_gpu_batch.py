@@ -150,3 +150,23 @@ def test_time_limits_report_timeouts(ctx):
     for q in qs:
         q.free()
     g.free()
+
+
+def test_autotune_picks_a_valid_setting(ctx):
+    hg = csm.gen_rmat(12, 40000, 6, 1.0, 33)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    r = random.Random(44)
+    plans = [csm.gen_query(host, r.randint(3, 12), csm.QUERY_AVG_DEGREE, 700 + i, avg_degree=r.uniform(2, 3.5)).plan()
+             for i in range(20)]
+    qs = [g.prepare(p) for p in plans]
+    cap = 300_000
+    tuned = csm.batch_autotune(g, qs, cap=cap, sample=8, time_limit_ms=20000)
+    assert 1 <= tuned.wgs_per_cu <= 8 and 1 <= tuned.max_out <= 8 and 32 <= tuned.block_kb <= 4096
+    assert 1 <= tuned.ilp <= 4 and tuned.cap == cap
+    res = csm.batch_match(g, qs, opts=tuned)
+    want = [(x.count, x.capped) for x in (q.match(cap=cap)[0] for q in qs)]
+    assert [(x.count, x.capped) for x in res] == want
+    for q in qs:
+        q.free()
+    g.free()
