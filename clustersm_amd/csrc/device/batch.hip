@@ -1155,7 +1155,7 @@ csm_status BatchRun::launch(hipStream_t s) {
                 o.ilp ? std::min(o.ilp, kIlp) : kIlp, o.max_out ? std::min(o.max_out, kMaxOut) : 1u};
   // with a time limit the queries are bounded by time; otherwise by items, so
   // an unlucky depth-first search is handed to the plain path and stays exact
-  const uint64_t budget = o.item_budget ? o.item_budget : o.time_limit_ms ? ~0ull : 2000000000ull;
+  const uint64_t budget = o.item_budget ? o.item_budget : o.time_limit_ms ? ~0ull : 5000000000ull;
   if (o.chunk_rows) tp.root_words = std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, o.chunk_rows / 64));
   if (tp.wgs_per_cu > 8 || tp.block_kb < 32 || tp.block_kb > 4096) return set_error(ctx, CSM_E_ARG, "batch tuning out of range");
   const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;

@@ -253,7 +253,7 @@ typedef struct csm_batch_opts {
   uint32_t time_limit_ms;  /* 0: none (the kernel still stops after 120 s and the plain path finishes
                               its queries, exact); else per query: a query unfinished after this has
                               status CSM_E_TIMEOUT and its count so far, as with csm_match */
-  uint64_t item_budget;    /* items per query before the plain path takes it over; 0 = 2e9 without a
+  uint64_t item_budget;    /* items per query before the plain path takes it over; 0 = 5e9 without a
                               time limit, unlimited with one */
 } csm_batch_opts;
 csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
