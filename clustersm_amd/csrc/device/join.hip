@@ -59,12 +59,26 @@ struct ExpandArgs {
   uint32_t *out;            // MATERIALIZE: rows of k + 1 ids
   unsigned long long *count;  // COUNT / FUSED total
   LevelJoin lv_last;        // FUSED: the last position (k + 1)
-  const uint64_t *cand_last;
+  // FUSED with the last position's parent in an earlier column: per row, the
+  // list of that parent's id and its length minus the row's own members
+  const uint32_t *fbase;
+  const uint64_t *fbeg;
+  const uint32_t *flen;
+  uint32_t fused_rows;
+  uint32_t last_inj_k;      // the last position's label equals position k's
 };
 
 enum { kMark = 0, kCount = 1, kMaterialize = 2, kFused = 3 };
 
-__device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
+// v in the sorted list cn[beg .. beg + len)
+__device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint64_t len, uint32_t v) {
+  while (len > 1) {
+    const uint64_t half = len >> 1;
+    if (cn[beg + half] <= v) beg += half;
+    len -= half;
+  }
+  return len == 1 && cn[beg] == v;
+}
 
 template <typename T>
 __device__ inline T block_sum(T v, T *lds) {
@@ -125,6 +139,7 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   uint32_t w = 0;
   const uint32_t *row = nullptr;
   unsigned long long contrib = 0;
+  uint64_t rsel = 0;  // the item's row: rlo + rsel (staged) or rsel (wide)
   if (valid) {
     const uint64_t gi = a.item_base + item;
     uint32_t d;
@@ -134,11 +149,13 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
         const uint32_t mid = (lo + hi) >> 1;
         if (sd[mid] <= gi) lo = mid; else hi = mid;
       }
+      rsel = lo;
       row = s_rows + lo * k;
       d = a.drv[rlo + lo];
       w = a.ci.cn_adj[s_base[lo] + (gi - sd[lo])];
     } else {
       const uint64_t r = upper_row(a.dscan, rlo, rend, gi);
+      rsel = r;
       row = a.rows + r * k;
       d = a.drv[r];
       uint64_t beg, end;
@@ -153,17 +170,25 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
       for (uint32_t t = 0; ok && t < a.lv.nanchor; ++t)
         if (t != d) ok = a.probe.has(row[a.lv.anchor[t]], w);
       if (MODE == kFused && ok) {
+        // the last position's candidates adjacent to its parent x, minus the
+        // row's vertices among them (w in C(u_last) and w ~ x iff w is in x's list)
         const uint32_t pk = a.lv_last.parent;
-        const uint32_t x = pk == k ? w : row[pk];
-        uint64_t lb, le;
-        a.ci.list(a.lv_last, 0, x, lb, le);
-        long long c = static_cast<long long>(le - lb);
-        for (uint32_t t = 0; t < a.lv_last.ninj; ++t) {
-          const uint32_t j = a.lv_last.inj[t];
-          const uint32_t v = j == k ? w : row[j];
-          if (v != x && bit_of(a.cand_last, v) && a.probe.has(x, v)) --c;
+        if (a.fused_rows) {  // x = row[pk]: per-row constant from the row pass
+          const uint64_t r = (wide ? 0 : rlo) + rsel;
+          long long c = static_cast<long long>(a.fbase[r]);
+          if (a.last_inj_k && w != row[pk] && in_list(a.ci.cn_adj, a.fbeg[r], a.flen[r], w)) --c;
+          contrib = static_cast<unsigned long long>(c);
+        } else {  // x = w
+          uint64_t lb, le;
+          a.ci.list(a.lv_last, 0, w, lb, le);
+          long long c = static_cast<long long>(le - lb);
+          for (uint32_t t = 0; t < a.lv_last.ninj; ++t) {
+            const uint32_t j = a.lv_last.inj[t];
+            if (j == k) continue;
+            if (in_list(a.ci.cn_adj, lb, le - lb, row[j])) --c;
+          }
+          contrib = static_cast<unsigned long long>(c);
         }
-        contrib = static_cast<unsigned long long>(c);
       }
     }
   }
@@ -197,20 +222,22 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   }
 }
 
-// *out += sum of vals[0..n) (one workgroup; avoids same-address atomics per tile).
-__global__ __launch_bounds__(1024) void reduce_add_kernel(const uint64_t *vals, uint64_t n, unsigned long long *out) {
-  __shared__ unsigned long long lds[1024 / kWave];
+// *out += sum of vals[0..n): a few workgroups (one atomic each, not one per tile).
+__global__ __launch_bounds__(256) void reduce_add_kernel(const uint64_t *vals, uint64_t n, unsigned long long *out) {
+  __shared__ unsigned long long lds[256 / kWave];
   unsigned long long acc = 0;
-  for (uint64_t i = threadIdx.x; i < n; i += 1024) acc += vals[i];
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) acc += vals[i];
   for (int d = kWave / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, kWave);
   if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
-    for (int w = 0; w < 1024 / kWave; ++w) t += lds[w];
-    *out += t;
+    for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
+    if (t) atomicAdd(out, t);
   }
 }
+
+uint32_t reduce_grid(uint64_t n) { return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(256, n / 2048))); }
 
 // tile_row[t] = row (relative to a) holding item t * kTile of the chunk
 // [a, b); tile_row[ntiles] = b - a.
@@ -223,28 +250,81 @@ __global__ void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, 
   if (blockIdx.x == 0 && threadIdx.x == 0) tile_row[ntiles] = static_cast<uint32_t>(b - a);
 }
 
+// Per-row pass of a level.
 // dscan[i] = shortest candidate list among the row's anchors, drv[i] = that
 // anchor's index; dscan[n] = 0.
-__global__ void anchor_len_kernel(CandIndex ci, const uint32_t *rows, uint32_t k, LevelJoin lv, uint64_t n,
-                                  uint64_t *dscan, uint8_t *drv) {
+// At the count level, when the last position's parent is an earlier column
+// (fused_rows): fbase/fbeg/flen[i] = that parent's list of the last position
+// and its length minus the row's own members.  With row_count (one anchor,
+// and no per-item correction), the row's count is its driver list minus the
+// row's own ids, times fbase when fused: it goes to partials[block] and the
+// row gets no items (dscan[i] = 0).
+struct RowPass {
+  LevelJoin lv_last;
+  uint32_t fused_rows, row_count;
+  uint32_t *fbase;
+  uint64_t *fbeg;
+  uint32_t *flen;
+  uint64_t *partials;  // row_count: one per workgroup
+};
+
+__global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uint32_t *rows, uint32_t k, LevelJoin lv,
+                                                         uint64_t n, uint64_t *dscan, uint8_t *drv, RowPass rp) {
+  __shared__ unsigned long long lds[256 / kWave];
+  unsigned long long acc = 0;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i <= n; i += 256ull * gridDim.x) {
     if (i == n) {
       dscan[i] = 0;
       continue;
     }
     const uint32_t *row = rows + i * k;
-    uint64_t best = ~0ull;
+    uint64_t best = ~0ull, bbeg = 0;
     uint32_t bi = 0;
     for (uint32_t t = 0; t < lv.nanchor; ++t) {
       uint64_t beg, end;
       ci.list(lv, t, row[lv.anchor[t]], beg, end);
       if (end - beg < best) {
         best = end - beg;
+        bbeg = beg;
         bi = t;
       }
     }
+    uint32_t fb = 1;
+    if (rp.fused_rows) {
+      const uint32_t xv = row[rp.lv_last.parent];
+      uint64_t lb, le;
+      ci.list(rp.lv_last, 0, xv, lb, le);
+      uint64_t c = le - lb;
+      for (uint32_t t = 0; t < rp.lv_last.ninj; ++t) {
+        const uint32_t col = rp.lv_last.inj[t];
+        if (col == k) continue;
+        const uint32_t v = row[col];
+        if (v != xv && in_list(ci.cn_adj, lb, le - lb, v)) --c;
+      }
+      fb = static_cast<uint32_t>(c);
+      rp.fbase[i] = fb;
+      rp.fbeg[i] = lb;
+      rp.flen[i] = static_cast<uint32_t>(le - lb);
+    }
+    if (rp.row_count) {
+      uint64_t surv = best;
+      for (uint32_t t = 0; t < lv.ninj; ++t)
+        if (in_list(ci.cn_adj, bbeg, best, row[lv.inj[t]])) --surv;
+      acc += surv * fb;
+      best = 0;
+    }
     dscan[i] = best;
     drv[i] = static_cast<uint8_t>(bi);
+  }
+  if (rp.row_count) {
+    for (int d = kWave / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
+      rp.partials[blockIdx.x] = t;
+    }
   }
 }
 
@@ -303,6 +383,8 @@ struct Workspace {
   uint8_t *drv[CSM_MAX_QV + 1] = {};
   uint64_t *mask = nullptr, *tiles = nullptr, *partials = nullptr, *ctl = nullptr;
   uint32_t *tile_row = nullptr;
+  uint32_t *fbase = nullptr, *flen = nullptr;  // count level, per row
+  uint64_t *fbeg = nullptr, *row_sums = nullptr;
   ~Workspace() {
     for (void *p : blocks) ctx->arena.free(p);
   }
@@ -332,6 +414,8 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   const bool fused = precount && L >= 2;     // counted from level L-1 items
   const bool rows2 = precount && L == 1;     // two-vertex query: count level-1 rows
   const uint32_t stop = fused ? L - 1 : L;   // deepest level whose items are expanded
+  // fused, last position's parent before `stop`: its list is a per-row constant
+  const bool fused_rows = fused && lv[L].parent != stop;
 
   const uint64_t n1 = q->counts[lv[0].u];
   res.count = 0;
@@ -350,13 +434,19 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   uint64_t per_item = 16;  // mask bit, tile counter, scan partials, rounded up
   for (uint32_t k = 2; k <= stop; ++k) per_item += 4ull * k + 8 + 1;
   if (want_list) per_item += 4ull * nq;
+  if (fused_rows) per_item += 16;
   // candidate adjacency index first: its size is data dependent
   CandIndexHost cih;
   csm_status st = build_cand_index(ctx, g, q, cih, ws.blocks, kt);
   if (st != CSM_OK) return st;
   CandIndex ci{q->bits, cih.rank_pref, cih.cn_off, cih.cn_adj, words};
   const LevelJoin *lj = cih.level;
-  const uint64_t fixed = 13ull * (n1 + 1) + 8ull * (words + 1) + (1 << 20);
+  // the count level's rows count outright when one anchor drives them and no
+  // per-item correction applies (the new vertex cannot be in the last list)
+  bool last_inj_k = false;
+  for (uint32_t t = 0; t < lj[L].ninj; ++t) last_inj_k |= lj[L].inj[t] == stop;
+  const bool row_count = fused_rows && !last_inj_k && lj[stop].nanchor == 1;
+  const uint64_t fixed = 13ull * (n1 + 1) + 8ull * (words + 1) + (fused_rows ? 16ull * (n1 + 1) : 0) + (2 << 20);
   const uint64_t avail = ctx->arena.largest_free();
   if (avail < fixed + per_item * g->max_degree * 2)
     return set_error(ctx, CSM_E_NOMEM, "arena too small for the join workspace");
@@ -385,9 +475,16 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     ws.rows[nq] = static_cast<uint32_t *>(ws.get(4ull * nq * item_cap));
     okalloc = ws.rows[nq] != nullptr;
   }
+  if (okalloc && fused_rows) {
+    const uint64_t nr = (stop == 1 ? n1 : item_cap) + 1;
+    ws.fbase = static_cast<uint32_t *>(ws.get(4 * nr));
+    ws.flen = static_cast<uint32_t *>(ws.get(4 * nr));
+    ws.fbeg = static_cast<uint64_t *>(ws.get(8 * nr));
+    ws.row_sums = static_cast<uint64_t *>(ws.get(8 * 65536));
+    okalloc = ws.fbase && ws.flen && ws.fbeg && ws.row_sums;
+  }
   if (!okalloc) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (join workspace)");
   CSM_HIP_TRY(ctx, hipMemsetAsync(ws.ctl, 0, 8 * kCtlSize, s));
-  const uint64_t *cand_last = q->bits + static_cast<uint64_t>(lv[L].u) * words;
   uint64_t *pin = ctx->pinned;
   if (rows2) {  // two vertices: the count is the total length of the one slot's lists
     CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, cih.cn_off + lj[1].off_base[0], 8, hipMemcpyDeviceToHost, s));
@@ -416,14 +513,46 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   uint64_t next_b[CSM_MAX_QV + 1] = {}, next_items[CSM_MAX_QV + 1] = {}, next_base[CSM_MAX_QV + 1] = {};
   nrows[1] = n1;
 
+  std::vector<uint32_t> host_rows;
+  uint64_t total = 0;
+  uint32_t k = 1;
+  bool stop_all = false;
+
   // driver-degree prefix for level k and the first chunk choice
   auto prepare_level = [&](uint32_t k) -> csm_status {
+    RowPass rp{};
+    const bool count_level = k == stop && fused_rows;
+    if (count_level) {
+      rp.lv_last = lj[L];
+      rp.fused_rows = 1;
+      rp.row_count = row_count;
+      rp.fbase = ws.fbase;
+      rp.fbeg = ws.fbeg;
+      rp.flen = ws.flen;
+      rp.partials = ws.row_sums;
+    }
+    const uint32_t grid = grid_for(nrows[k] + 1);
+    // per row: the row, a rank word + prefix + two offsets per list looked up
+    const uint64_t per_row = 4ull * k + 24ull * (lv[k].nanchor + (count_level ? 1 : 0));
+    if (count_level && row_count) {  // the whole level is counted here: no items
+      kt.begin(CSM_K_COUNT);
+      anchor_len_kernel<<<grid, 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k], ws.drv[k], rp);
+      reduce_add_kernel<<<reduce_grid(grid), 256, 0, s>>>(ws.row_sums, grid,
+                                                          reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCount));
+      kt.end(CSM_K_COUNT, nrows[k] * per_row);
+      CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
+      CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+      res.items += nrows[k];
+      total = pin[kCtlCount];
+      if (total > cap) stop_all = true;
+      cursor[k] = nrows[k];
+      return CSM_OK;
+    }
     kt.begin(CSM_K_AUX);
-    anchor_len_kernel<<<grid_for(nrows[k] + 1), 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k],
-                                                            ws.drv[k]);
+    anchor_len_kernel<<<grid, 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k], ws.drv[k], rp);
     scan_u64_exclusive(ws.dscan[k], nrows[k] + 1, ws.partials, nullptr, s);
     choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], 0, item_cap, ws.ctl + kCtlChunkA);
-    kt.end(CSM_K_AUX, nrows[k] * (4ull * lv[k].nanchor + 8 * lv[k].nanchor + 17));
+    kt.end(CSM_K_AUX, nrows[k] * (per_row + 9 + (count_level ? 16 : 0)));
     CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     cursor[k] = 0;
@@ -443,13 +572,9 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     return CSM_OK;
   };
 
+  const auto deadline = t0 + std::chrono::milliseconds(o.time_limit_ms);
   st = prepare_level(1);
   if (st != CSM_OK) return st;
-  std::vector<uint32_t> host_rows;
-  uint64_t total = 0;
-  uint32_t k = 1;
-  bool stop_all = false;
-  const auto deadline = t0 + std::chrono::milliseconds(o.time_limit_ms);
   while (!stop_all) {
     if (o.time_limit_ms && std::chrono::steady_clock::now() > deadline) {
       res.status = CSM_E_TIMEOUT;  // count so far is a lower bound
@@ -499,7 +624,11 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     if (k == stop && !want_list) {
       if (fused) {
         ea.lv_last = lj[L];
-        ea.cand_last = cand_last;
+        ea.fused_rows = fused_rows;
+        ea.last_inj_k = last_inj_k;
+        ea.fbase = ws.fbase;
+        ea.fbeg = ws.fbeg;
+        ea.flen = ws.flen;
         kt.begin(CSM_K_COUNT);
         expand_kernel<kFused><<<ntiles, kTile, lds_rows, s>>>(ea);
         kt.end(CSM_K_COUNT, step_in);
@@ -508,7 +637,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
         expand_kernel<kCount><<<ntiles, kTile, lds_rows, s>>>(ea);
         kt.end(CSM_K_COUNT, step_in);
       }
-      reduce_add_kernel<<<1, 1024, 0, s>>>(ws.tiles, ntiles, ea.count);
+      reduce_add_kernel<<<reduce_grid(ntiles), 256, 0, s>>>(ws.tiles, ntiles, ea.count);
       cursor[k] = b;
       st = next_chunk(k, b, true);
       if (st != CSM_OK) return st;
