@@ -535,11 +535,11 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     // per row: the row, a rank word + prefix + two offsets per list looked up
     const uint64_t per_row = 4ull * k + 24ull * (lv[k].nanchor + (count_level ? 1 : 0));
     if (count_level && row_count) {  // the whole level is counted here: no items
-      kt.begin(CSM_K_COUNT);
+      kt.begin(CSM_K_AUX);
       anchor_len_kernel<<<grid, 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k], ws.drv[k], rp);
       reduce_add_kernel<<<reduce_grid(grid), 256, 0, s>>>(ws.row_sums, grid,
                                                           reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCount));
-      kt.end(CSM_K_COUNT, nrows[k] * per_row);
+      kt.end(CSM_K_AUX, nrows[k] * per_row);
       CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
       CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
       res.items += nrows[k];
