@@ -1115,7 +1115,7 @@ csm_status BatchRun::prepare(hipStream_t s) {
     for (uint32_t k = 0; k < nq; ++k) lvs[static_cast<size_t>(i) * CSM_MAX_QV + k] = cih_[i].level[k];
     BQuery &b = bq[i];
     std::memset(&b, 0, sizeof(b));
-    b.ci = CandIndex{q->bits, cih_[i].rank_pref, cih_[i].cn_off, cih_[i].cn_adj, words};
+    b.ci = CandIndex{q->bits, cih_[i].rank_pref, cih_[i].cn_off, cih_[i].cn_len, cih_[i].cn_adj, words};
     b.lv = d_lv + static_cast<size_t>(i) * CSM_MAX_QV;
     b.root_bits = q->bits + static_cast<uint64_t>(q->level[0].u) * words;
     b.cand_last = q->bits + static_cast<uint64_t>(q->level[L].u) * words;
@@ -1132,10 +1132,8 @@ csm_status BatchRun::prepare(hipStream_t s) {
   }
   for (auto &tv : two_vertex) {
     const uint64_t n1 = queries_[tv.first]->counts[queries_[tv.first]->level[0].u];
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, cih_[tv.first].cn_off + tv.second, 8, hipMemcpyDeviceToHost, s));
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned + 1, cih_[tv.first].cn_off + tv.second + n1, 8, hipMemcpyDeviceToHost, s));
-    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-    host_count_[tv.first] = ctx->pinned[1] - ctx->pinned[0];
+    csm_status st = slot_total(ctx, cih_[tv.first], tv.second, n1, host_count_[tv.first]);
+    if (st != CSM_OK) return st;
   }
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_lv, lvs.data(), sizeof(LevelJoin) * lvs.size(), hipMemcpyHostToDevice, s));
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_bq_, bq.data(), sizeof(BQuery) * nqs, hipMemcpyHostToDevice, s));

@@ -1,10 +1,13 @@
 // Build of the candidate adjacency index (see cand_index.hpp).
 //
-// Two passes over the candidates of every slot (query edge s -> t): count
-// |N(x) ∩ C(t)| into the slot's offset array at x's rank, one exclusive scan
-// over all slots at once (offsets become absolute positions in cn_adj), then
-// write the lists, each sorted.
+// For every slot (query edge s -> t) and x in C(s): a bound on |N(x) ∩ C(t)|
+// from the neighbour-label table (x's neighbours of label(t); x's degree
+// when the byte saturates or the label is beyond the table) goes to x's rank
+// in the slot's offset array; one exclusive scan over all slots at once turns
+// the bounds into list starts in cn_adj; one pass over the adjacency then
+// writes every list, sorted, and its exact length.
 #include <algorithm>
+#include <vector>
 
 #include "cand_index.hpp"
 
@@ -21,6 +24,8 @@ struct SlotDesc {
   uint32_t src, dst;
   uint64_t base;    // first entry of the slot in cn_off
   uint64_t ncand;   // |C(src)|
+  uint32_t dst_label;
+  uint32_t pad;
 };
 
 struct RankJob {
@@ -43,22 +48,48 @@ __global__ void rank_words_kernel(const RankJob *jobs, uint32_t words) {
 
 __device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
 
+// List bounds: per slot and x in C(s) (blockIdx.y = slot), the number of
+// x's neighbours labelled label(t), else deg(x); a zero closes each slot.
+__global__ __launch_bounds__(kThreads) void bound_kernel(const uint32_t *__restrict__ off,
+                                                         const uint8_t *__restrict__ nlc, uint32_t nlc_labels,
+                                                         uint32_t words, const SlotDesc *__restrict__ slots,
+                                                         uint64_t *__restrict__ cn_off) {
+  const SlotDesc sd = slots[blockIdx.y];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint64_t *xrow = sd.bits + static_cast<uint64_t>(sd.src) * words;
+  const uint64_t *pref = sd.rank_pref + static_cast<uint64_t>(sd.src) * (words + 1);
+  const uint64_t pref0 = pref[0];
+  const bool table = nlc != nullptr && sd.dst_label < nlc_labels;
+  if (blockIdx.x == 0 && threadIdx.x == 0) cn_off[sd.base + sd.ncand] = 0;
+  for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {
+    const uint64_t xw = xrow[b];
+    if (!((xw >> lane) & 1ull)) continue;
+    const uint32_t x = b * 64u + static_cast<uint32_t>(lane);
+    uint32_t bound = off[x + 1] - off[x];
+    if (table) {
+      const uint32_t c = nlc[static_cast<uint64_t>(x) * kNlcStride + sd.dst_label];
+      if (c < 255) bound = min(bound, c);
+    }
+    cn_off[sd.base + (pref[b] - pref0) + lanes_below(xw)] = bound;
+  }
+}
+
 // A wave takes one 64-bit word of C(s): the adjacency ranges of its set
 // lanes are contiguous in the CSR order of the concatenation [lane 0's
 // neighbours, lane 1's, ...], which the wave walks 64 entries at a time
 // (coalesced loads, no divergence on skewed degrees).  Each entry finds its
-// owner lane by binary search over the degree prefix.  COUNT: hits per
-// owner, LDS atomics.  FILL: an owner's hits are written at its list start
-// plus the hits before them (earlier chunks: a running count; this chunk:
-// a ballot restricted to the owner's lanes), so every list stays sorted.
-template <bool FILL>
+// owner lane by binary search over the degree prefix.  An owner's hits are
+// written at its list start plus the hits before them (earlier chunks: a
+// running count in LDS; this chunk: a ballot restricted to the owner's
+// lanes), so every list stays sorted; the final count is the list's length.
 __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                                                       uint32_t words, const SlotDesc *__restrict__ slots,
-                                                      uint64_t *__restrict__ cn_off, uint32_t *__restrict__ cn_adj) {
+                                                      const uint64_t *__restrict__ cn_off, uint32_t *__restrict__ cn_len,
+                                                      uint32_t *__restrict__ cn_adj) {
   __shared__ uint32_t s_pre[kWaves][kWave + 1];  // exclusive prefix of the lanes' degrees (0 for unset lanes)
   __shared__ uint32_t s_o0[kWaves][kWave];       // adjacency start per lane
   __shared__ uint32_t s_run[kWaves][kWave];      // hits so far per lane
-  __shared__ uint64_t s_start[kWaves][kWave];    // FILL: list start per lane
+  __shared__ uint64_t s_start[kWaves][kWave];    // list start per lane
   const SlotDesc sd = slots[blockIdx.y];
   const uint64_t *bits = sd.bits;
   const uint64_t *rank_pref = sd.rank_pref;
@@ -68,7 +99,6 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
   const uint64_t *pref = rank_pref + static_cast<uint64_t>(sd.src) * (words + 1);
   const uint64_t pref0 = pref[0];
   uint32_t *pre = s_pre[wave];
-  if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cn_off[sd.base + sd.ncand] = 0;
   for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {
     const uint64_t xw = xrow[b];
     if (xw == 0) continue;
@@ -90,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
     if (lane == kWave - 1) pre[kWave] = total;
     s_o0[wave][lane] = o0;
     s_run[wave][lane] = 0;
-    if (FILL) s_start[wave][lane] = act ? cn_off[slot] : 0;
+    s_start[wave][lane] = act ? cn_off[slot] : 0;
     wave_sync();
     for (uint32_t base = 0; base < total; base += kWave) {
       const uint32_t e = base + static_cast<uint32_t>(lane);
@@ -106,19 +136,17 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
         w = adj[s_o0[wave][owner] + (e - pre[owner])];
         hit = bit_of(yrow, w);
       }
-      if (FILL) {
-        const uint64_t m = ballot(hit);
-        if (hit) {
-          const uint32_t first = max(base, pre[owner]) - base;  // owner's first lane in this chunk
-          const uint64_t seg = m & ~((1ull << first) - 1ull);
-          cn_adj[s_start[wave][owner] + s_run[wave][owner] + lanes_below(seg)] = w;
-        }
-        wave_sync();
+      const uint64_t m = ballot(hit);
+      if (hit) {
+        const uint32_t first = max(base, pre[owner]) - base;  // owner's first lane in this chunk
+        const uint64_t seg = m & ~((1ull << first) - 1ull);
+        cn_adj[s_start[wave][owner] + s_run[wave][owner] + lanes_below(seg)] = w;
       }
+      wave_sync();
       if (hit) atomicAdd(&s_run[wave][owner], 1u);
       wave_sync();
     }
-    if (!FILL && act) cn_off[slot] = s_run[wave][lane];
+    if (act) cn_len[slot] = s_run[wave][lane];
     wave_sync();
   }
 }
@@ -171,18 +199,19 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
         J.anchor[a] = L.anchor[a];
         J.src_u[a] = src;
         J.off_base[a] = entries;
-        slots.push_back({q->bits, ci.rank_pref, src, L.u, entries, q->counts[src]});
+        slots.push_back({q->bits, ci.rank_pref, src, L.u, entries, q->counts[src], q->desc.label[L.u], 0});
         entries += q->counts[src] + 1;
       }
     }
   }
   const uint64_t scan_n = std::max<uint64_t>(rank_n, entries);
   uint64_t *cn_off = static_cast<uint64_t *>(get(8 * std::max<uint64_t>(entries, 1)));
+  uint32_t *cn_len = static_cast<uint32_t *>(get(4 * std::max<uint64_t>(entries, 1)));
   uint64_t *partials = static_cast<uint64_t *>(get(8 * (scan_n / kScanTile + 2)));
   uint64_t *total_d = static_cast<uint64_t *>(get(64));
   SlotDesc *d_slots = static_cast<SlotDesc *>(get(sizeof(SlotDesc) * std::max<size_t>(slots.size(), 1)));
   RankJob *d_jobs = static_cast<RankJob *>(get(sizeof(RankJob) * nqs));
-  if (!cn_off || !partials || !total_d || !d_slots || !d_jobs)
+  if (!cn_off || !cn_len || !partials || !total_d || !d_slots || !d_jobs)
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate index)");
   std::vector<RankJob> jobs(nqs);
   for (uint32_t i = 0; i < nqs; ++i) jobs[i] = {qs[i]->bits, out[i].rank_pref, qs[i]->desc.nq};
@@ -192,7 +221,10 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   scan_u64_exclusive(rank_all, rank_n, partials, nullptr, s);
   kt.end(CSM_K_AUX, 16ull * rank_n);
   if (slots.empty()) return CSM_OK;
-  for (uint32_t i = 0; i < nqs; ++i) out[i].cn_off = cn_off;
+  for (uint32_t i = 0; i < nqs; ++i) {
+    out[i].cn_off = cn_off;
+    out[i].cn_len = cn_len;
+  }
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_slots, slots.data(), sizeof(SlotDesc) * slots.size(), hipMemcpyHostToDevice, s));
   const uint32_t gx = std::max<uint32_t>(1, std::min<uint32_t>(std::max<uint32_t>(8, 1024 / nqs), ceil_div(words, kWaves)));
   uint64_t cand_words = 0;
@@ -200,8 +232,8 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {  // grid.y limit
     const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
     kt.begin(CSM_K_AUX);
-    cn_kernel<false><<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, nullptr);
-    kt.end(CSM_K_AUX, 16ull * words * ns);
+    bound_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->nlc, g->nlc_labels, words, d_slots + s0, cn_off);
+    kt.end(CSM_K_AUX, 8ull * words * ns + 9ull * cand_words * ns / slots.size());
   }
   scan_u64_exclusive(cn_off, entries, partials, total_d, s);
   CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, total_d, 8, hipMemcpyDeviceToHost, s));
@@ -212,7 +244,7 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {
     const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
     kt.begin(CSM_K_AUX);
-    cn_kernel<true><<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, cn_adj);
+    cn_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, cn_len, cn_adj);
     kt.end(CSM_K_AUX, 8ull * words * ns + 4ull * total * ns / slots.size());
   }
   for (uint32_t i = 0; i < nqs; ++i) {
@@ -220,6 +252,16 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
     out[i].total_lists = total;
   }
   out[0].build_bytes = 24ull * words * slots.size() + 8ull * cand_words + 16ull * entries + 4ull * total;
+  return CSM_OK;
+}
+
+csm_status slot_total(csm_ctx *ctx, const CandIndexHost &ci, uint64_t base, uint64_t n, uint64_t &total) {
+  total = 0;
+  if (n == 0) return CSM_OK;
+  std::vector<uint32_t> len(n);
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(len.data(), ci.cn_len + base, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (uint32_t l : len) total += l;
   return CSM_OK;
 }
 

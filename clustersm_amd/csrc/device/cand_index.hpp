@@ -23,7 +23,8 @@ struct LevelJoin {
 struct CandIndex {
   const uint64_t *bits;       // nq x words candidate bitmaps
   const uint64_t *rank_pref;  // nq x (words + 1): exclusive popcount prefix, global over all rows
-  const uint64_t *cn_off;     // per slot |C(s)| + 1 absolute list offsets into cn_adj
+  const uint64_t *cn_off;     // per slot and x in C(s): start of x's list in cn_adj
+  const uint32_t *cn_len;     // and its length
   const uint32_t *cn_adj;
   uint32_t words;
 
@@ -36,15 +37,16 @@ struct CandIndex {
   }
   // [begin, end) of N(x) ∩ C(u_k) for anchor a of position k
   __device__ inline void list(const LevelJoin &L, uint32_t a, uint32_t x, uint64_t &beg, uint64_t &end) const {
-    const uint64_t *o = cn_off + L.off_base[a] + rank(L.src_u[a], x);
-    beg = o[0];
-    end = o[1];
+    const uint64_t i = L.off_base[a] + rank(L.src_u[a], x);
+    beg = cn_off[i];
+    end = beg + cn_len[i];
   }
 };
 
 struct CandIndexHost {
   uint64_t *rank_pref = nullptr;
   uint64_t *cn_off = nullptr;
+  uint32_t *cn_len = nullptr;
   uint32_t *cn_adj = nullptr;
   uint64_t total_lists = 0;   // entries of cn_adj
   uint64_t build_bytes = 0;   // algorithmic bytes of the build
@@ -55,6 +57,10 @@ struct CandIndexHost {
 // by `blocks` (freed by the caller).  Launches on ctx->stream and synchronises.
 csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
                             std::vector<void *> &blocks, KernelTimer &kt);
+
+// Total length of the n lists of a slot starting at entry `base` (host copy
+// of their lengths on ctx->stream; synchronises).
+csm_status slot_total(csm_ctx *ctx, const CandIndexHost &ci, uint64_t base, uint64_t n, uint64_t &total);
 
 // All queries' indexes with one launch per pass (one shared list array).
 csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,

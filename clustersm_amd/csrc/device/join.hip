@@ -439,7 +439,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   CandIndexHost cih;
   csm_status st = build_cand_index(ctx, g, q, cih, ws.blocks, kt);
   if (st != CSM_OK) return st;
-  CandIndex ci{q->bits, cih.rank_pref, cih.cn_off, cih.cn_adj, words};
+  CandIndex ci{q->bits, cih.rank_pref, cih.cn_off, cih.cn_len, cih.cn_adj, words};
   const LevelJoin *lj = cih.level;
   // the count level's rows count outright when one anchor drives them and no
   // per-item correction applies (the new vertex cannot be in the last list)
@@ -487,11 +487,10 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   CSM_HIP_TRY(ctx, hipMemsetAsync(ws.ctl, 0, 8 * kCtlSize, s));
   uint64_t *pin = ctx->pinned;
   if (rows2) {  // two vertices: the count is the total length of the one slot's lists
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, cih.cn_off + lj[1].off_base[0], 8, hipMemcpyDeviceToHost, s));
-    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin + 1, cih.cn_off + lj[1].off_base[0] + n1, 8, hipMemcpyDeviceToHost, s));
-    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    uint64_t total2 = 0;
+    st = slot_total(ctx, cih, lj[1].off_base[0], n1, total2);
+    if (st != CSM_OK) return st;
     kt.flush();
-    const uint64_t total2 = pin[1] - pin[0];
     res.count = std::min<uint64_t>(total2, cap);
     res.capped = total2 > cap;
     res.items = n1;
