@@ -1,0 +1,56 @@
+// Runs the library, built for the CPU emulation, on graph files:
+//   driver <data.graph> <cap> <query.graph>...
+// prints "plain <count> <capped>" and "batch <count> <capped>" per query.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "csm.h"
+
+static void check(csm_status s, const char *what, csm_ctx *ctx) {
+  if (s != CSM_OK) {
+    std::fprintf(stderr, "%s: status %d %s\n", what, s, ctx ? csm_last_error(ctx) : csm_host_last_error());
+    std::exit(1);
+  }
+}
+
+int main(int argc, char **argv) {
+  if (argc < 4) return 2;
+  const uint64_t cap = std::strtoull(argv[2], nullptr, 10);
+  csm_ctx *ctx = nullptr;
+  csm_host_graph *hd = nullptr;
+  check(csm_host_graph_read(argv[1], &hd), "read data", nullptr);
+  check(csm_open(0, 1ull << 30, &ctx), "open", nullptr);
+  csm_graph *g = nullptr;
+  check(csm_graph_upload(ctx, hd, &g), "upload", ctx);
+  std::vector<csm_query *> qs;
+  static csm_query_plan plan;
+  for (int i = 3; i < argc; ++i) {
+    csm_host_graph *hq = nullptr;
+    check(csm_host_graph_read(argv[i], &hq), "read query", nullptr);
+    check(csm_plan_build(hq, &plan), "plan", nullptr);
+    csm_query *q = nullptr;
+    check(csm_query_prepare(ctx, g, &plan, &q), "prepare", ctx);
+    qs.push_back(q);
+    csm_host_graph_free(hq);
+  }
+  for (csm_query *q : qs) {
+    csm_match_opts o{};
+    o.cap = cap;
+    o.refine = 1;
+    csm_result r{};
+    check(csm_match(ctx, g, q, &o, &r), "match", ctx);
+    std::printf("plain %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+  }
+  csm_batch_opts bo{};
+  bo.cap = cap;
+  bo.use_graph = 1;
+  std::vector<csm_result> res(qs.size());
+  check(csm_batch_match(ctx, g, qs.data(), static_cast<uint32_t>(qs.size()), &bo, res.data()), "batch", ctx);
+  for (const csm_result &r : res) std::printf("batch %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+  for (csm_query *q : qs) csm_query_free(ctx, q);
+  csm_graph_free(ctx, g);
+  csm_host_graph_free(hd);
+  csm_close(ctx);
+  return 0;
+}

@@ -1,0 +1,70 @@
+"""The device code on the CPU, under AddressSanitizer.
+
+tests/emu/build.py compiles the library's HIP sources for the host: each
+workgroup runs on an OS thread and its lanes on fibers, with block barriers
+and wave operations as meeting points (tests/emu/emu_rt.cpp).  Every global
+allocation is its own heap block, so an out-of-bounds access in a kernel is
+reported with its stack.  The plain path and the persistent batched kernel
+run on small random graphs and queries (with non-tree edges) and must give
+the oracle's counts, capped and uncapped.  Test infrastructure only; the
+product is the gfx950 build."""
+import os
+import random
+import subprocess
+import sys
+
+import pytest
+
+import clustersm_amd as csm
+from helpers import rand_graph, rand_query
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def driver():
+    sys.path.insert(0, os.path.join(HERE, "emu"))
+    import build
+    try:
+        return str(build.build())
+    except (OSError, subprocess.CalledProcessError) as e:
+        pytest.skip(f"g++ with AddressSanitizer is not available here: {e}")
+
+
+def _oracle(dl, de, queries, cap):
+    d = O.Data.from_arrays(dl, de)
+    out = []
+    for ql, qe in queries:
+        oq = O.Query.from_arrays(ql, qe)
+        c, capped, _, _, _ = O.match(d, oq, oq.plan(), cap=cap if cap else (1 << 64) - 1, threads=4)
+        out.append((c, bool(capped)))
+    return out
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_device_code_on_cpu_matches_oracle(driver, tmp_path, seed):
+    r = random.Random(seed)
+    n = r.randint(60, 140)
+    nl = r.randint(1, 3)
+    dl, de = rand_graph(r, n, r.uniform(0.05, 0.12), nl)
+    queries = [rand_query(r, r.randint(3, 7), r.randint(0, 3), nl) for _ in range(6)]
+    dpath = tmp_path / "data.graph"
+    csm.HostGraph.from_edges(dl, de).write(str(dpath))
+    qpaths = []
+    for i, (ql, qe) in enumerate(queries):
+        p = tmp_path / f"q{i}.graph"
+        csm.HostGraph.from_edges(ql, qe).write(str(p))
+        qpaths.append(str(p))
+    env = dict(os.environ, EMU_THREADS="4", EMU_CUS="2", ASAN_OPTIONS="detect_leaks=0")
+    for cap in (0, 50):
+        out = subprocess.run([driver, str(dpath), str(cap), *qpaths], env=env, capture_output=True, text=True,
+                             timeout=600)
+        assert out.returncode == 0, out.stderr[-3000:]
+        assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
+        lines = out.stdout.split("\n")
+        plain = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("plain")]
+        batch = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("batch")]
+        want = _oracle(dl, de, queries, cap)
+        assert plain == want, (cap, plain, want)
+        assert batch == want, (cap, batch, want)
