@@ -176,7 +176,9 @@ def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0):
     t_batch = time.perf_counter() - t2
     ctx.set_profiling(False)
     print(f"# batched: {len(keep)} queries in {t_batch:.2f}s", file=sys.stderr, flush=True)
-    equal = all((x.count, x.capped, x.status) == (plain[i].count, plain[i].capped, 0) for x, i in zip(res, keep))
+    mism = [[i, x.count, int(x.capped), x.status, x.path, plain[i].count, int(plain[i].capped)]
+            for x, i in zip(res, keep) if (x.count, x.capped, x.status) != (plain[i].count, plain[i].capped, 0)]
+    equal = not mism
     paths = {}
     for x in res:
         paths[x.path] = paths.get(x.path, 0) + 1
@@ -190,6 +192,7 @@ def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0):
         "plain_queries_per_s": round(len(keep) / t_plain, 2),
         "speedup_vs_plain": round(t_plain / t_batch, 2),
         "counts_equal_plain": equal,
+        "mismatches": mism,
         "autotuned": {"wgs_per_cu": tuned.wgs_per_cu, "max_out": tuned.max_out, "block_kb": tuned.block_kb,
                       "ilp": tuned.ilp, "sample_queries": min(64, len(keep)), "tuning_s": round(tune_s, 1)},
         "excluded": {"queries": dropped, "rule": f"the plain path does not finish them within {limit_ms} ms "
@@ -280,7 +283,7 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
     csm.batch_match(g, [wq], cap=cap)  # warm-up (code objects)
     wq.free()
     lat = [0.0] * len(keep)
-    pending, nxt, done, sizes, equal = [], 0, 0, [], True
+    pending, nxt, done, sizes, mism = [], 0, 0, [], []
     budget = budget_gb * (1 << 30)
     ts = time.perf_counter()
     while done < len(keep):
@@ -305,7 +308,8 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
         for j, x, q in zip(batch, res, qs):
             lat[j] = tend - arr[j]
             p = plain[keep[j]]
-            equal = equal and (x.status, x.count, x.capped) == (0, p.count, p.capped)
+            if (x.status, x.count, x.capped) != (0, p.count, p.capped):
+                mism.append([keep[j], x.count, int(x.capped), x.status, x.path, p.count, int(p.capped), len(batch)])
             q.free()
         sizes.append(len(batch))
         done += len(batch)
@@ -327,7 +331,8 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
         "plain_fifo": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
                        "queries_per_s": round(len(keep) / max(free_at - arr[0], 1e-9), 2),
                        "mean_service_ms": round(1e3 * sum(service[i] for i in keep) / max(1, len(keep)), 1)},
-        "counts_equal_plain": equal,
+        "counts_equal_plain": not mism,
+        "mismatches": mism,
     }
 
 

@@ -1280,7 +1280,9 @@ csm_status BatchRun::finish(csm_result *results) {
   // kernel stopped early, are finished by the plain path (one at a time,
   // exact), except that with a caller's time limit a query still unfinished
   // when the limit passed keeps its count so far with status CSM_E_TIMEOUT.
-  uint32_t fallbacks = 0;
+  // A running query whose task accounting did not close (tasks pending, or
+  // one dropped) is finished by the plain path too: its count is not trusted.
+  uint32_t fallbacks = 0, unclosed = 0;
   const double ms_kernel_end = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
   for (uint32_t i = 0; i < nqs; ++i) {
     if (!active_[i] || hst[i].done == kCapped) continue;
@@ -1290,7 +1292,8 @@ csm_status BatchRun::finish(csm_result *results) {
       results[i].status = CSM_E_TIMEOUT;
       continue;
     }
-    if (hst[i].done == kAbandoned || hctl.abort) {
+    {
+      if (hst[i].done == kRunning && !hctl.abort) ++unclosed;
       csm_match_opts mo{};
       mo.cap = o_.cap;
       mo.refine = o_.refine;
@@ -1320,7 +1323,9 @@ csm_status BatchRun::finish(csm_result *results) {
     results[i].ms_total = ms_all;
     results[i].ms_join = ms_all - ms_filter_;
   }
-  if (trace) std::fprintf(stderr, "csm batch: %u queries finished by the plain path, %.1f ms in all\n", fallbacks, ms_all);
+  if (trace || unclosed)
+    std::fprintf(stderr, "csm batch: %u queries finished by the plain path (%u with unclosed task accounting), %.1f ms in all\n",
+                 fallbacks, unclosed, ms_all);
   return CSM_OK;
 }
 
