@@ -28,6 +28,7 @@
 // results are exact either way.
 // Counts only: a capped list needs the plain path's ordered enumeration.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -70,15 +71,19 @@ struct BState {
 };
 
 enum : unsigned int { kRunning = 0, kCapped = 1, kAbandoned = 2 };
-enum : unsigned int { kAbortPool = 1, kAbortTime = 2 };
+enum : unsigned int { kAbortPool = 1, kAbortTime = 2, kAbortBadTask = 3 };
 
 // A task record: one per pool block (a block's rows form at most one task)
 // and one per root range.
-struct Task {
-  uint32_t q, k, block, r0, r1, skip_rows, skip_items, pad;
+// `seq` tags each write of a record; the deque entry that names the record
+// carries the same tag, so a reader can tell the record it expects from an
+// older one at the same index.
+struct alignas(16) Task {
+  uint32_t q, k, block, r0, r1, skip_rows, skip_items, seq;
 };
 
 constexpr uint32_t kNil = 0xFFFFFFFFu;
+constexpr unsigned long long kNilEntry = ~0ull;  // deque entries: record id | seq << 32
 constexpr uint32_t kDequeCap = 1024;  // > levels x (held blocks + 2): a deque never fills
 
 // One task deque per workgroup, split in two: entries [top, pub) are
@@ -116,8 +121,10 @@ struct BCtl {
   unsigned long long sync_ticks;   // real-time ticks in deque and free-ring operations
   unsigned long long phase_ticks[4];  // rows staged + drivers, count tiles, extension tiles, task ends
   unsigned long long alloc_fail;   // consecutive failed allocations from the free ring
-  unsigned int abort;              // kAbortPool or kAbortTime once the kernel stops early
+  unsigned int abort;              // kAbortPool, kAbortTime or kAbortBadTask once the kernel stops early
   unsigned int overflow;           // a deque was full (never expected)
+  unsigned long long late_recs;    // records whose tag matched only after a re-read (never expected)
+  unsigned long long bad_recs;     // records never matching their entry, or out of range (never expected)
 };
 
 struct BArgs {
@@ -127,7 +134,8 @@ struct BArgs {
   BCtl *ctl;
   Task *recs;                      // num_blocks + root tasks
   Deque *dq;                       // one per workgroup
-  uint32_t *dq_buf;                // kDequeCap record ids per workgroup
+  unsigned long long *dq_buf;      // kDequeCap entries per workgroup
+  uint32_t epoch;                  // differs between launches: part of every record tag
   uint32_t nqueries;
   uint32_t tasks_per_query;        // root tasks per query
   uint32_t words;                  // bitmap words
@@ -173,20 +181,20 @@ __device__ inline void dunlock(unsigned int *l) {
   __hip_atomic_store(l, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Thief: oldest record of workgroup v, or kNil.  A thief counted idle
+// Thief: oldest entry of workgroup v, or kNilEntry.  A thief counted idle
 // leaves the idle count while it holds v's lock: v can only declare itself
 // idle after the unlock, so the count never reaches every workgroup while a
 // task is in hand.
-__device__ uint32_t deque_steal(const BArgs &a, uint32_t v, bool counted_idle) {
+__device__ unsigned long long deque_steal(const BArgs &a, uint32_t v, bool counted_idle) {
   Deque *d = &a.dq[v];
-  if (ld_rlx(&d->pub) <= ld_rlx(&d->top)) return kNil;
-  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(v) * kDequeCap;
+  if (ld_rlx(&d->pub) <= ld_rlx(&d->top)) return kNilEntry;
+  unsigned long long *buf = a.dq_buf + static_cast<uint64_t>(v) * kDequeCap;
   dlock(&d->lock);
   // acquire: the entries below pub, their records and rows were written
   // before the owner's release of pub
   const unsigned long long b = __hip_atomic_load(&d->pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   const unsigned long long t = ld_rlx(&d->top);
-  uint32_t r = kNil;
+  unsigned long long r = kNilEntry;
   if (b > t) {
     r = buf[t % kDequeCap];
     st_rlx(&d->top, t + 1);
@@ -255,6 +263,7 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     c.phase_ticks[0] = c.phase_ticks[1] = c.phase_ticks[2] = c.phase_ticks[3] = 0;
     c.alloc_fail = 0;
     c.abort = c.overflow = 0;
+    c.late_recs = c.bad_recs = 0;
   }
 }
 
@@ -291,7 +300,8 @@ struct Smem {
   uint32_t nheld;                   // full output blocks of the current task, in row order
   uint32_t held[kMaxOut];
   uint32_t held_fill[kMaxOut];
-  uint32_t push[kMaxOut + 2];       // records pushed when the task ends
+  unsigned long long push[kMaxOut + 2];  // entries pushed when the task ends
+  uint32_t seq_ctr;                 // record writes by this workgroup
   uint32_t cache[kCache];           // free blocks owned by this workgroup
   uint32_t ncache;
   unsigned long long dq_bottom;     // owner's end of its deque
@@ -340,11 +350,11 @@ __device__ void deque_publish(const BArgs &a, Smem &sm, uint32_t wg, unsigned lo
   sm.dq_pub = upto;
 }
 
-// Owner: push n records in order (the last one ends on top), privately;
+// Owner: push n entries in order (the last one ends on top), privately;
 // published at once when a workgroup is idle.
-__device__ void deque_push(const BArgs &a, Smem &sm, uint32_t wg, const uint32_t *recs, uint32_t n) {
+__device__ void deque_push(const BArgs &a, Smem &sm, uint32_t wg, const unsigned long long *recs, uint32_t n) {
   if (n == 0) return;
-  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
+  unsigned long long *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
   const unsigned long long b = sm.dq_bottom, t = ld_rlx(&a.dq[wg].top);
   if (b - t + n > kDequeCap) {
     atomicExch(&a.ctl->overflow, 1u);
@@ -356,19 +366,20 @@ __device__ void deque_push(const BArgs &a, Smem &sm, uint32_t wg, const uint32_t
   if (ld_rlx(&a.ctl->idle) > 0) deque_publish(a, sm, wg, sm.dq_bottom);
 }
 
-// Owner: newest record, or kNil.  Private entries need no synchronisation;
-// while others are idle, all but the one taken are published first.
-__device__ uint32_t deque_pop(const BArgs &a, Smem &sm, uint32_t wg) {
-  uint32_t *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
+// Owner: newest entry, or kNilEntry.  Private entries need no
+// synchronisation; while others are idle, all but the one taken are
+// published first.
+__device__ unsigned long long deque_pop(const BArgs &a, Smem &sm, uint32_t wg) {
+  unsigned long long *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
   if (sm.dq_bottom > sm.dq_pub) {
     if (sm.dq_bottom - sm.dq_pub > 1 && ld_rlx(&a.ctl->idle) > 0) deque_publish(a, sm, wg, sm.dq_bottom - 1);
     return buf[(--sm.dq_bottom) % kDequeCap];
   }
   Deque *d = &a.dq[wg];
-  if (sm.dq_pub <= ld_rlx(&d->top)) return kNil;
+  if (sm.dq_pub <= ld_rlx(&d->top)) return kNilEntry;
   dlock(&d->lock);
   const unsigned long long t = ld_rlx(&d->top), p = sm.dq_pub;
-  uint32_t r = kNil;
+  unsigned long long r = kNilEntry;
   if (p > t) {
     r = buf[(p - 1) % kDequeCap];
     st_rlx(&d->pub, p - 1);
@@ -469,7 +480,10 @@ __device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, u
   return f[0];
 }
 
-__device__ inline void write_rec(const BArgs &a, uint32_t rec, const Task &t) {
+// Thread 0: writes task t into record `rec` under a new tag and returns the
+// deque entry that names it.
+__device__ inline unsigned long long write_rec(const BArgs &a, Smem &sm, uint32_t wg, uint32_t rec, const Task &t) {
+  const uint32_t seq = ((sm.seq_ctr++ & 0x1FFFFFu) | (wg << 21)) ^ a.epoch;
   Task &r = a.recs[rec];
   r.q = t.q;
   r.k = t.k;
@@ -478,6 +492,42 @@ __device__ inline void write_rec(const BArgs &a, uint32_t rec, const Task &t) {
   r.r1 = t.r1;
   r.skip_rows = t.skip_rows;
   r.skip_items = t.skip_items;
+  r.seq = seq;
+  return (static_cast<unsigned long long>(seq) << 32) | rec;
+}
+
+// Thread 0: the record an entry names, once its tag matches (re-read after
+// an acquire while it does not, a bounded number of times), checked against
+// the bounds every later access relies on.  False: the kernel must stop.
+__device__ bool read_rec(const BArgs &a, unsigned long long ent, Task &t) {
+  const uint32_t rec = static_cast<uint32_t>(ent), seq = static_cast<uint32_t>(ent >> 32);
+  if (rec >= a.num_blocks + static_cast<uint64_t>(a.nqueries) * a.tasks_per_query) return false;
+  unsigned long long *w = reinterpret_cast<unsigned long long *>(&a.recs[rec]);
+  for (uint32_t spin = 0;; ++spin) {
+    unsigned long long v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t.q = static_cast<uint32_t>(v[0]);
+    t.k = static_cast<uint32_t>(v[0] >> 32);
+    t.block = static_cast<uint32_t>(v[1]);
+    t.r0 = static_cast<uint32_t>(v[1] >> 32);
+    t.r1 = static_cast<uint32_t>(v[2]);
+    t.skip_rows = static_cast<uint32_t>(v[2] >> 32);
+    t.skip_items = static_cast<uint32_t>(v[3]);
+    t.seq = static_cast<uint32_t>(v[3] >> 32);
+    if (t.seq == seq) {
+      if (spin) atomicAdd(&a.ctl->late_recs, 1ull);
+      break;
+    }
+    if (spin >= (1u << 16)) return false;
+    acquire_agent();
+    __builtin_amdgcn_s_sleep(2);
+  }
+  if (t.q >= a.nqueries || t.k == 0 || t.k >= a.qs[t.q].nq) return false;
+  if (t.k == 1)
+    return t.r0 < t.r1 && t.r1 <= a.words && t.r1 - t.r0 <= a.root_words && t.skip_rows <= 64u * (t.r1 - t.r0);
+  return t.block < a.num_blocks && t.r0 <= t.r1 && static_cast<uint64_t>(t.r1) * t.k <= a.block_words &&
+         t.skip_rows <= t.r1 - t.r0;
 }
 
 // Process one task; returns with all threads converged.
@@ -807,24 +857,21 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     if (!live && ld_done(a, t.q) == kRunning) a.st[t.q].lost = 1;  // its outputs are dropped by an early stop
     uint32_t np = 0;
     if (sm.flag != 0 && live) {  // same rows, resume point updated
-      write_rec(a, sm.rec, sm.task);
-      sm.push[np++] = sm.rec;
+      sm.push[np++] = write_rec(a, sm, wg, sm.rec, sm.task);
     } else if (k >= 2) {
       blk_free(a, sm, t.block);
     }
     // lexicographically first rows on top: the last block is pushed first
     if (sm.out_block != kNil) {
       if (fill && live) {
-        write_rec(a, sm.out_block, Task{t.q, kout, sm.out_block, 0, fill, 0, 0, 0});
-        sm.push[np++] = sm.out_block;
+        sm.push[np++] = write_rec(a, sm, wg, sm.out_block, Task{t.q, kout, sm.out_block, 0, fill, 0, 0, 0});
       } else {
         blk_free(a, sm, sm.out_block);
       }
     }
     for (int i = static_cast<int>(sm.nheld) - 1; i >= 0; --i) {
       if (live) {
-        write_rec(a, sm.held[i], Task{t.q, kout, sm.held[i], 0, sm.held_fill[i], 0, 0, 0});
-        sm.push[np++] = sm.held[i];
+        sm.push[np++] = write_rec(a, sm, wg, sm.held[i], Task{t.q, kout, sm.held[i], 0, sm.held_fill[i], 0, 0, 0});
       } else {
         blk_free(a, sm, sm.held[i]);
       }
@@ -878,6 +925,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
     sm.staged_q = kNil;
     sm.staged_k = kNil;
     sm.ncache = 0;
+    sm.seq_ctr = 0;
     sm.dq_bottom = sm.dq_pub = 0;
     sm.acc_q = kNil;
     sm.acc_count = sm.acc_items = sm.acc_rows = 0;
@@ -891,9 +939,9 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       sm.state = 1;
       const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
       // own deque, then a new root task, then the oldest task of another workgroup
-      uint32_t rec = idle ? kNil : deque_pop(a, sm, wg);  // an idle workgroup's deque is empty
-      bool root = false;
-      if (rec == kNil && ld_rlx(&a.ctl->root_next) < nroot) {
+      unsigned long long ent = idle ? kNilEntry : deque_pop(a, sm, wg);  // an idle workgroup's deque is empty
+      uint32_t rec = kNil;
+      if (ent == kNilEntry && ld_rlx(&a.ctl->root_next) < nroot) {
         if (idle) {
           atomicAdd(&a.ctl->idle, ~0ull);
           idle = false;
@@ -906,22 +954,30 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
           if (ld_done(a, q) != kRunning) continue;
           rec = static_cast<uint32_t>(a.num_blocks + static_cast<uint64_t>(q) * a.tasks_per_query + c);
           sm.task = Task{q, 1, 0, c * a.root_words, min(a.words, (c + 1) * a.root_words), 0, 0, 0};
-          root = true;
           break;
         }
       }
-      if (rec == kNil) {
-        for (uint32_t i = 0; i < 8 && rec == kNil; ++i) {
+      if (rec == kNil && ent == kNilEntry) {
+        for (uint32_t i = 0; i < 8 && ent == kNilEntry; ++i) {
           const uint32_t v = xorshift(rng) % nwgs;
-          if (v != wg) rec = deque_steal(a, v, idle);
+          if (v != wg) ent = deque_steal(a, v, idle);
         }
-        if (rec != kNil) {
+        if (ent != kNilEntry) {
           ++n_steals;
           idle = false;
         }
       }
+      if (ent != kNilEntry) {  // a record from a deque: its tag and bounds first
+        Task t;
+        if (read_rec(a, ent, t)) {
+          sm.task = t;
+          rec = static_cast<uint32_t>(ent);
+        } else {  // never expected: stop, and the host recounts unfinished queries exactly
+          atomicAdd(&a.ctl->bad_recs, 1ull);
+          atomicCAS(&a.ctl->abort, 0u, kAbortBadTask);
+        }
+      }
       if (rec != kNil) {
-        if (!root) sm.task = a.recs[rec];
         sm.rec = rec;
         sm.state = 0;
         sm.stop = ld_done(a, sm.task.q) != kRunning || aborted(a);
@@ -1170,7 +1226,7 @@ csm_status BatchRun::launch(hipStream_t s) {
   uint32_t *pool = static_cast<uint32_t *>(get(4 * block_words * num_blocks));
   Task *recs = static_cast<Task *>(get(sizeof(Task) * (num_blocks + nroot)));
   Deque *dq = static_cast<Deque *>(get(sizeof(Deque) * grid));
-  uint32_t *dq_buf = static_cast<uint32_t *>(get(4ull * kDequeCap * grid));
+  unsigned long long *dq_buf = static_cast<unsigned long long *>(get(8ull * kDequeCap * grid));
   FreeSlot *free_ring = static_cast<FreeSlot *>(get(sizeof(FreeSlot) * num_blocks));
   unsigned long long *ctr = static_cast<unsigned long long *>(get(8ull * 2));
   d_ctl_ = get(sizeof(BCtl));
@@ -1184,6 +1240,8 @@ csm_status BatchRun::launch(hipStream_t s) {
   a.recs = recs;
   a.dq = dq;
   a.dq_buf = dq_buf;
+  static std::atomic<uint32_t> launches{0};
+  a.epoch = (launches.fetch_add(1) + 1) * 0x9E3779B1u;
   a.nqueries = nqs;
   a.tasks_per_query = tasks_per_query;
   a.words = words;
@@ -1260,6 +1318,9 @@ csm_status BatchRun::finish(csm_result *results) {
                  hctl.phase_ticks[0] / 1e5 / grid, hctl.phase_ticks[1] / 1e5 / grid, hctl.phase_ticks[2] / 1e5 / grid,
                  hctl.phase_ticks[3] / 1e5 / grid);
   }
+  if (hctl.late_recs || hctl.bad_recs || hctl.abort == kAbortBadTask)
+    std::fprintf(stderr, "csm batch: %llu task records seen late, %llu never matched or out of range (abort %u)\n",
+                 hctl.late_recs, hctl.bad_recs, hctl.abort);
   for (uint32_t i = 0; i < nqs; ++i) {
     csm_result &r = results[i];
     std::memset(&r, 0, sizeof(r));
