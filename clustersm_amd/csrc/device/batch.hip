@@ -209,7 +209,7 @@ __device__ void free_push(const BArgs &a, uint32_t id) {  // bounded MPMC ring, 
   FreeSlot *s = &a.free_ring[pos % a.num_blocks];
   for (uint32_t spin = 0; ld_rlx(&s->seq) != 2 * pos && spin < kMaxSpin; ++spin) __builtin_amdgcn_s_sleep(1);
   acquire_agent();
-  s->id = id;
+  __hip_atomic_store(&s->id, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   st_rel(&s->seq, 2 * pos + 1);
 }
 
@@ -222,7 +222,7 @@ __device__ bool free_pop(const BArgs &a, uint32_t &id) {
     if (diff == 0) {
       if (atomicCAS(a.free_head, pos, pos + 1) == pos) {
         acquire_agent();
-        id = s->id;
+        id = __hip_atomic_load(&s->id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         st_rel(&s->seq, 2 * (pos + a.num_blocks));
         return true;
       }
@@ -236,35 +236,46 @@ __device__ bool free_pop(const BArgs &a, uint32_t &id) {
 }
 
 // Initial state: every block in the free ring, deques empty, counters zero.
+// Every field the join kernel synchronises on is stored with an agent-scope
+// atomic, so the join kernel's atomics see this launch's state and never a
+// value an earlier launch left at the same address (the arena hands the same
+// blocks to consecutive calls).
+template <typename T>
+__device__ inline void st_init(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x, stride = 256ull * gridDim.x;
   for (uint64_t i = tid; i < a.num_blocks; i += stride) {
-    a.free_ring[i].id = static_cast<uint32_t>(i);
-    a.free_ring[i].seq = 2 * i + 1;
+    st_init(&a.free_ring[i].id, static_cast<uint32_t>(i));
+    st_init(&a.free_ring[i].seq, static_cast<unsigned long long>(2 * i + 1));
   }
   for (uint64_t i = tid; i < nwgs; i += stride) {
-    a.dq[i].lock = 0;
-    a.dq[i].top = 0;
-    a.dq[i].pub = 0;
+    st_init(&a.dq[i].lock, 0u);
+    st_init(&a.dq[i].top, 0ull);
+    st_init(&a.dq[i].pub, 0ull);
   }
   for (uint64_t i = tid; i < a.nqueries; i += stride) {
-    a.st[i].count = 0;
-    a.st[i].done = a.qs[i].active ? kRunning : kCapped;
-    a.st[i].lost = 0;
-    a.st[i].pending = a.qs[i].active ? a.tasks_per_query : 0;
-    a.st[i].items = a.st[i].rows = 0;
+    st_init(&a.st[i].count, 0ull);
+    st_init(&a.st[i].done, a.qs[i].active ? static_cast<unsigned int>(kRunning) : static_cast<unsigned int>(kCapped));
+    st_init(&a.st[i].lost, 0u);
+    st_init(&a.st[i].pending, a.qs[i].active ? static_cast<unsigned long long>(a.tasks_per_query) : 0ull);
+    st_init(&a.st[i].items, 0ull);
+    st_init(&a.st[i].rows, 0ull);
   }
   if (tid == 0) {
-    *a.free_head = 0;
-    *a.free_tail = a.num_blocks;
+    st_init(a.free_head, 0ull);
+    st_init(a.free_tail, static_cast<unsigned long long>(a.num_blocks));
     BCtl &c = *a.ctl;
-    c.root_next = c.idle = c.started = c.items = c.rows_in = c.rows_out = c.tasks = c.steals = c.fails = 0;
-    c.busy_ticks = c.idle_ticks = c.sync_ticks = 0;
-    c.phase_ticks[0] = c.phase_ticks[1] = c.phase_ticks[2] = c.phase_ticks[3] = 0;
-    c.alloc_fail = 0;
-    c.abort = c.overflow = 0;
-    c.late_recs = c.bad_recs = 0;
+    unsigned long long *c64[] = {&c.root_next, &c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out, &c.tasks,
+                                 &c.steals, &c.fails, &c.busy_ticks, &c.idle_ticks, &c.sync_ticks, &c.phase_ticks[0],
+                                 &c.phase_ticks[1], &c.phase_ticks[2], &c.phase_ticks[3], &c.alloc_fail,
+                                 &c.late_recs, &c.bad_recs};
+    for (unsigned long long *f : c64) st_init(f, 0ull);
+    st_init(&c.abort, 0u);
+    st_init(&c.overflow, 0u);
   }
+  __threadfence();
 }
 
 constexpr uint32_t kIlp = 4;     // items per thread per tile, checked in lockstep
@@ -483,7 +494,7 @@ __device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, u
 // Thread 0: writes task t into record `rec` under a new tag and returns the
 // deque entry that names it.
 __device__ inline unsigned long long write_rec(const BArgs &a, Smem &sm, uint32_t wg, uint32_t rec, const Task &t) {
-  const uint32_t seq = ((sm.seq_ctr++ & 0x1FFFFFu) | (wg << 21)) ^ a.epoch;
+  const uint32_t seq = ((a.epoch & 0xFFu) << 24) | ((wg & 0x7FFu) << 13) | (sm.seq_ctr++ & 0x1FFFu);
   Task &r = a.recs[rec];
   r.q = t.q;
   r.k = t.k;
@@ -501,6 +512,8 @@ __device__ inline unsigned long long write_rec(const BArgs &a, Smem &sm, uint32_
 // the bounds every later access relies on.  False: the kernel must stop.
 __device__ bool read_rec(const BArgs &a, unsigned long long ent, Task &t) {
   const uint32_t rec = static_cast<uint32_t>(ent), seq = static_cast<uint32_t>(ent >> 32);
+  // an entry written by an earlier launch at the same address
+  if ((seq >> 24) != (a.epoch & 0xFFu)) return false;
   if (rec >= a.num_blocks + static_cast<uint64_t>(a.nqueries) * a.tasks_per_query) return false;
   unsigned long long *w = reinterpret_cast<unsigned long long *>(&a.recs[rec]);
   for (uint32_t spin = 0;; ++spin) {
@@ -1241,7 +1254,7 @@ csm_status BatchRun::launch(hipStream_t s) {
   a.dq = dq;
   a.dq_buf = dq_buf;
   static std::atomic<uint32_t> launches{0};
-  a.epoch = (launches.fetch_add(1) + 1) * 0x9E3779B1u;
+  a.epoch = launches.fetch_add(1) + 1;
   a.nqueries = nqs;
   a.tasks_per_query = tasks_per_query;
   a.words = words;
@@ -1263,12 +1276,14 @@ csm_status BatchRun::launch(hipStream_t s) {
   num_blocks_ = num_blocks;
   const size_t lds_rows = 4ull * kRowBatch * std::max<uint32_t>(1, max_nq_ - 1);
   auto launch_join = [&]() {
-    batch_init_kernel<<<grid_for(std::max<uint64_t>(num_blocks, grid)), 256, 0, s>>>(a, grid);
     if (tp.wgs_per_cu >= 4) batch_join_kernel<4><<<grid, kThreads, lds_rows, s>>>(a);
     else batch_join_kernel<2><<<grid, kThreads, lds_rows, s>>>(a);
   };
   join_kt_.reset(new KernelTimer(ctx, kstat_.data()));
   KernelTimer &kt = *join_kt_;
+  // the state reset is an ordinary launch before the join (and before its
+  // graph): a kernel boundary on the stream, not a node boundary in a graph
+  batch_init_kernel<<<grid_for(std::max<uint64_t>(num_blocks, grid)), 256, 0, s>>>(a, grid);
   if (o.use_graph) {  // the graph lives until finish() has waited for its launch
     CSM_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     launch_join();
