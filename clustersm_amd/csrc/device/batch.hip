@@ -125,6 +125,14 @@ struct BCtl {
   unsigned int overflow;           // a deque was full (never expected)
   unsigned long long late_recs;    // records whose tag matched only after a re-read (never expected)
   unsigned long long bad_recs;     // records never matching their entry, or out of range (never expected)
+  unsigned int bad_why;            // first bad record: 1 earlier epoch, 2 id range, 3 tag never matched, 4 bounds
+  unsigned int bad_wg;             // and the workgroup that read it
+  unsigned long long bad_ent;      // its entry
+  Task bad_task;                   // and the record as read
+  unsigned int bad_src, bad_victim;  // where the entry came from (0 own private, 1 own locked, 2 stolen)
+  unsigned long long bad_t, bad_b;   // slot index and bound it was taken at
+  unsigned int late_src, late_victim;  // the same for the first late record
+  unsigned long long late_t, late_b;
 };
 
 struct BArgs {
@@ -185,7 +193,8 @@ __device__ inline void dunlock(unsigned int *l) {
 // leaves the idle count while it holds v's lock: v can only declare itself
 // idle after the unlock, so the count never reaches every workgroup while a
 // task is in hand.
-__device__ unsigned long long deque_steal(const BArgs &a, uint32_t v, bool counted_idle) {
+__device__ unsigned long long deque_steal(const BArgs &a, uint32_t v, bool counted_idle, unsigned long long &at,
+                                          unsigned long long &bound) {
   Deque *d = &a.dq[v];
   if (ld_rlx(&d->pub) <= ld_rlx(&d->top)) return kNilEntry;
   unsigned long long *buf = a.dq_buf + static_cast<uint64_t>(v) * kDequeCap;
@@ -196,7 +205,9 @@ __device__ unsigned long long deque_steal(const BArgs &a, uint32_t v, bool count
   const unsigned long long t = ld_rlx(&d->top);
   unsigned long long r = kNilEntry;
   if (b > t) {
-    r = buf[t % kDequeCap];
+    r = __hip_atomic_load(&buf[t % kDequeCap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    at = t;
+    bound = b;
     st_rlx(&d->top, t + 1);
     if (counted_idle) atomicAdd(&a.ctl->idle, ~0ull);
   }
@@ -313,6 +324,10 @@ struct Smem {
   uint32_t held_fill[kMaxOut];
   unsigned long long push[kMaxOut + 2];  // entries pushed when the task ends
   uint32_t seq_ctr;                 // record writes by this workgroup
+  uint32_t ent_src;                 // where the last entry came from: 0 own private, 1 own locked, 2 stolen
+  uint32_t task_stolen;             // the task in hand came from another workgroup's deque
+  uint32_t ent_victim;
+  unsigned long long ent_t, ent_b;  // the indices it was taken at (slot, bound)
   uint32_t cache[kCache];           // free blocks owned by this workgroup
   uint32_t ncache;
   unsigned long long dq_bottom;     // owner's end of its deque
@@ -357,6 +372,9 @@ __device__ void blk_free(const BArgs &a, Smem &sm, uint32_t id) {
 // workgroup wrote before (entries, records, rows, after the barrier that
 // ended their task) is released with them.
 __device__ void deque_publish(const BArgs &a, Smem &sm, uint32_t wg, unsigned long long upto) {
+  // every store of this wave complete before the release begins its L2
+  // write-back (the entries and records are agent-scope stores as well)
+  __builtin_amdgcn_s_waitcnt(0);
   __hip_atomic_store(&a.dq[wg].pub, upto, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   sm.dq_pub = upto;
 }
@@ -372,7 +390,8 @@ __device__ void deque_push(const BArgs &a, Smem &sm, uint32_t wg, const unsigned
     atomicCAS(&a.ctl->abort, 0u, kAbortPool);
     return;
   }
-  for (uint32_t i = 0; i < n; ++i) buf[(b + i) % kDequeCap] = recs[i];
+  for (uint32_t i = 0; i < n; ++i)
+    __hip_atomic_store(&buf[(b + i) % kDequeCap], recs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   sm.dq_bottom = b + n;
   if (ld_rlx(&a.ctl->idle) > 0) deque_publish(a, sm, wg, sm.dq_bottom);
 }
@@ -384,6 +403,9 @@ __device__ unsigned long long deque_pop(const BArgs &a, Smem &sm, uint32_t wg) {
   unsigned long long *buf = a.dq_buf + static_cast<uint64_t>(wg) * kDequeCap;
   if (sm.dq_bottom > sm.dq_pub) {
     if (sm.dq_bottom - sm.dq_pub > 1 && ld_rlx(&a.ctl->idle) > 0) deque_publish(a, sm, wg, sm.dq_bottom - 1);
+    sm.ent_src = 0;
+    sm.ent_t = sm.dq_bottom - 1;
+    sm.ent_b = sm.dq_pub;
     return buf[(--sm.dq_bottom) % kDequeCap];
   }
   Deque *d = &a.dq[wg];
@@ -392,7 +414,10 @@ __device__ unsigned long long deque_pop(const BArgs &a, Smem &sm, uint32_t wg) {
   const unsigned long long t = ld_rlx(&d->top), p = sm.dq_pub;
   unsigned long long r = kNilEntry;
   if (p > t) {
-    r = buf[(p - 1) % kDequeCap];
+    r = __hip_atomic_load(&buf[(p - 1) % kDequeCap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sm.ent_src = 1;
+    sm.ent_t = p - 1;
+    sm.ent_b = t;
     st_rlx(&d->pub, p - 1);
     sm.dq_pub = sm.dq_bottom = p - 1;
   }
@@ -495,26 +520,28 @@ __device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, u
 // deque entry that names it.
 __device__ inline unsigned long long write_rec(const BArgs &a, Smem &sm, uint32_t wg, uint32_t rec, const Task &t) {
   const uint32_t seq = ((a.epoch & 0xFFu) << 24) | ((wg & 0x7FFu) << 13) | (sm.seq_ctr++ & 0x1FFFu);
-  Task &r = a.recs[rec];
-  r.q = t.q;
-  r.k = t.k;
-  r.block = t.block;
-  r.r0 = t.r0;
-  r.r1 = t.r1;
-  r.skip_rows = t.skip_rows;
-  r.skip_items = t.skip_items;
-  r.seq = seq;
+  // agent-scope stores, read back with agent-scope loads (read_rec)
+  unsigned long long *w = reinterpret_cast<unsigned long long *>(&a.recs[rec]);
+  const unsigned long long v[4] = {t.q | (static_cast<unsigned long long>(t.k) << 32),
+                                   t.block | (static_cast<unsigned long long>(t.r0) << 32),
+                                   t.r1 | (static_cast<unsigned long long>(t.skip_rows) << 32),
+                                   t.skip_items | (static_cast<unsigned long long>(seq) << 32)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) __hip_atomic_store(w + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return (static_cast<unsigned long long>(seq) << 32) | rec;
 }
 
 // Thread 0: the record an entry names, once its tag matches (re-read after
 // an acquire while it does not, a bounded number of times), checked against
-// the bounds every later access relies on.  False: the kernel must stop.
-__device__ bool read_rec(const BArgs &a, unsigned long long ent, Task &t) {
+// the bounds every later access relies on.  0: usable; else why not (the
+// kernel must stop).
+__device__ uint32_t read_rec(const BArgs &a, unsigned long long ent, Task &t, bool &late) {
+  late = false;
   const uint32_t rec = static_cast<uint32_t>(ent), seq = static_cast<uint32_t>(ent >> 32);
+  t = Task{};
   // an entry written by an earlier launch at the same address
-  if ((seq >> 24) != (a.epoch & 0xFFu)) return false;
-  if (rec >= a.num_blocks + static_cast<uint64_t>(a.nqueries) * a.tasks_per_query) return false;
+  if ((seq >> 24) != (a.epoch & 0xFFu)) return 1;
+  if (rec >= a.num_blocks + static_cast<uint64_t>(a.nqueries) * a.tasks_per_query) return 2;
   unsigned long long *w = reinterpret_cast<unsigned long long *>(&a.recs[rec]);
   for (uint32_t spin = 0;; ++spin) {
     unsigned long long v[4];
@@ -529,18 +556,21 @@ __device__ bool read_rec(const BArgs &a, unsigned long long ent, Task &t) {
     t.skip_items = static_cast<uint32_t>(v[3]);
     t.seq = static_cast<uint32_t>(v[3] >> 32);
     if (t.seq == seq) {
-      if (spin) atomicAdd(&a.ctl->late_recs, 1ull);
+      late = spin != 0;
       break;
     }
-    if (spin >= (1u << 16)) return false;
+    if (spin >= (1u << 16)) return 3;
     acquire_agent();
     __builtin_amdgcn_s_sleep(2);
   }
-  if (t.q >= a.nqueries || t.k == 0 || t.k >= a.qs[t.q].nq) return false;
+  if (t.q >= a.nqueries || t.k == 0 || t.k >= a.qs[t.q].nq) return 4;
   if (t.k == 1)
-    return t.r0 < t.r1 && t.r1 <= a.words && t.r1 - t.r0 <= a.root_words && t.skip_rows <= 64u * (t.r1 - t.r0);
+    return t.r0 < t.r1 && t.r1 <= a.words && t.r1 - t.r0 <= a.root_words && t.skip_rows <= 64u * (t.r1 - t.r0) ? 0
+                                                                                                             : 4;
   return t.block < a.num_blocks && t.r0 <= t.r1 && static_cast<uint64_t>(t.r1) * t.k <= a.block_words &&
-         t.skip_rows <= t.r1 - t.r0;
+                 t.skip_rows <= t.r1 - t.r0
+             ? 0
+             : 4;
 }
 
 // Process one task; returns with all threads converged.
@@ -563,9 +593,11 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) last_inj_k |= sm.lv_last.inj[x] == k;
   // count-level rows whose count needs no per-item test
   const bool row_counted = count_level && nanchor == 1 && (!Q.fused || (fused_rows && !last_inj_k));
-  // rows of the task
+  // rows of the task; a stolen task's rows were written on another workgroup
+  // (possibly another XCD): staged with agent-scope loads
+  const bool stolen = sm.task_stolen != 0;
   uint32_t nrows;
-  const uint32_t *grows = nullptr;
+  uint32_t *grows = nullptr;
   if (k == 1) {
     // set bits of the root words, appended in order through a scan
     const uint32_t nw = t.r1 - t.r0;
@@ -611,7 +643,10 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     if (sm.stop) break;
     const uint32_t nb = min(kRowBatch, nrows - row0);
     for (uint32_t i = tid; i < nb * k; i += kThreads)
-      s_rows[i] = k == 1 ? sm.roots[row0 + i] : grows[static_cast<uint64_t>(row0) * k + i];
+      s_rows[i] = k == 1    ? sm.roots[row0 + i]
+                  : stolen ? __hip_atomic_load(&grows[static_cast<uint64_t>(row0) * k + i], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : grows[static_cast<uint64_t>(row0) * k + i];
     __syncthreads();
     // candidate lists of the kept anchors, one (row, anchor) pair per thread
     for (uint32_t p = tid; p < nb * nkept; p += kThreads) {
@@ -861,7 +896,9 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     if (sm.flag) break;
     __syncthreads();
   }
-  // finish: publish rows, push continuation and blocks, free the input
+  // finish: publish rows, push continuation and blocks, free the input;
+  // every lane's row stores complete before the fence's L2 write-back
+  __builtin_amdgcn_s_waitcnt(0);
   __threadfence();
   const unsigned long long tot_contrib = block_reduce(contrib, sm.red);
   const unsigned long long tot_items = block_reduce(items_seen, sm.red);
@@ -967,13 +1004,18 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
           if (ld_done(a, q) != kRunning) continue;
           rec = static_cast<uint32_t>(a.num_blocks + static_cast<uint64_t>(q) * a.tasks_per_query + c);
           sm.task = Task{q, 1, 0, c * a.root_words, min(a.words, (c + 1) * a.root_words), 0, 0, 0};
+          sm.task_stolen = 0;
           break;
         }
       }
       if (rec == kNil && ent == kNilEntry) {
         for (uint32_t i = 0; i < 8 && ent == kNilEntry; ++i) {
           const uint32_t v = xorshift(rng) % nwgs;
-          if (v != wg) ent = deque_steal(a, v, idle);
+          if (v != wg) {
+            ent = deque_steal(a, v, idle, sm.ent_t, sm.ent_b);
+            sm.ent_src = 2;
+            sm.ent_victim = v;
+          }
         }
         if (ent != kNilEntry) {
           ++n_steals;
@@ -982,11 +1024,29 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       }
       if (ent != kNilEntry) {  // a record from a deque: its tag and bounds first
         Task t;
-        if (read_rec(a, ent, t)) {
+        bool late;
+        const uint32_t why = read_rec(a, ent, t, late);
+        if (late && atomicAdd(&a.ctl->late_recs, 1ull) == 0) {
+          a.ctl->late_src = sm.ent_src;
+          a.ctl->late_victim = sm.ent_victim;
+          a.ctl->late_t = sm.ent_t;
+          a.ctl->late_b = sm.ent_b;
+        }
+        if (why == 0) {
           sm.task = t;
+          sm.task_stolen = sm.ent_src == 2;
           rec = static_cast<uint32_t>(ent);
         } else {  // never expected: stop, and the host recounts unfinished queries exactly
-          atomicAdd(&a.ctl->bad_recs, 1ull);
+          if (atomicAdd(&a.ctl->bad_recs, 1ull) == 0) {
+            a.ctl->bad_why = why;
+            a.ctl->bad_wg = wg;
+            a.ctl->bad_ent = ent;
+            a.ctl->bad_task = t;
+            a.ctl->bad_src = sm.ent_src;
+            a.ctl->bad_victim = sm.ent_victim;
+            a.ctl->bad_t = sm.ent_t;
+            a.ctl->bad_b = sm.ent_b;
+          }
           atomicCAS(&a.ctl->abort, 0u, kAbortBadTask);
         }
       }
@@ -1255,6 +1315,8 @@ csm_status BatchRun::launch(hipStream_t s) {
   a.dq_buf = dq_buf;
   static std::atomic<uint32_t> launches{0};
   a.epoch = launches.fetch_add(1) + 1;
+  launch_epoch_ = a.epoch;
+  tasks_per_query_ = tasks_per_query;
   a.nqueries = nqs;
   a.tasks_per_query = tasks_per_query;
   a.words = words;
@@ -1334,8 +1396,21 @@ csm_status BatchRun::finish(csm_result *results) {
                  hctl.phase_ticks[3] / 1e5 / grid);
   }
   if (hctl.late_recs || hctl.bad_recs || hctl.abort == kAbortBadTask)
-    std::fprintf(stderr, "csm batch: %llu task records seen late, %llu never matched or out of range (abort %u)\n",
-                 hctl.late_recs, hctl.bad_recs, hctl.abort);
+    std::fprintf(stderr,
+                 "csm batch: %llu task records seen late, %llu never matched or out of range (abort %u); first: "
+                 "reason %u, workgroup %u of %u, entry rec %u tag %08x (epoch %u), read q %u k %u block %u rows %u..%u "
+                 "skip %u/%u tag %08x; queries %u blocks %llu roots/query %u\n",
+                 hctl.late_recs, hctl.bad_recs, hctl.abort, hctl.bad_why, hctl.bad_wg, grid_,
+                 static_cast<uint32_t>(hctl.bad_ent), static_cast<uint32_t>(hctl.bad_ent >> 32),
+                 static_cast<uint32_t>(launch_epoch_ & 0xFF), hctl.bad_task.q, hctl.bad_task.k, hctl.bad_task.block,
+                 hctl.bad_task.r0, hctl.bad_task.r1, hctl.bad_task.skip_rows, hctl.bad_task.skip_items, hctl.bad_task.seq,
+                 nqs, static_cast<unsigned long long>(num_blocks_), tasks_per_query_);
+  if (hctl.late_recs || hctl.bad_recs)
+    std::fprintf(stderr,
+                 "csm batch: first bad entry from source %u (0 own private, 1 own locked, 2 stolen) victim %u "
+                 "at %llu bound %llu; first late record from source %u victim %u at %llu bound %llu\n",
+                 hctl.bad_src, hctl.bad_victim, hctl.bad_t, hctl.bad_b, hctl.late_src, hctl.late_victim, hctl.late_t,
+                 hctl.late_b);
   for (uint32_t i = 0; i < nqs; ++i) {
     csm_result &r = results[i];
     std::memset(&r, 0, sizeof(r));

@@ -41,6 +41,7 @@ class BatchRun {
   double ms_filter_ = 0;
   uint32_t words_ = 0, max_nq_ = 0, grid_ = 0;
   uint64_t num_blocks_ = 0;
+  uint32_t launch_epoch_ = 0, tasks_per_query_ = 0;
   std::vector<csm_kernel_stat> kstat_;
   std::vector<CandIndexHost> cih_;
   std::vector<uint8_t> active_;

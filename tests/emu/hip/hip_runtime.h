@@ -109,6 +109,7 @@ uint64_t emu_memrealtime();
 #define __builtin_amdgcn_fence(o, s) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_wave_barrier() (void)emu_wave_meet(__LINE__, 0, nullptr)
 #define __builtin_amdgcn_s_sleep(n) emu_yield()
+#define __builtin_amdgcn_s_waitcnt(n) ((void)0)
 #define __builtin_amdgcn_s_memrealtime() emu_memrealtime()
 #define __lane_id() (emu_lane() & 63u)
 
