@@ -118,9 +118,6 @@ def test_batch_tuning_variants_agree(ctx):
     g.free()
 
 
-@pytest.mark.skip(reason="its last GPU run ended in a GPU memory fault whose cause is not found yet (a CPU "
-                         "emulation of the same graph, queries, caps and limits under AddressSanitizer is clean); "
-                         "kept off the GPU until the cause is shown")
 def test_time_limits_report_timeouts(ctx):
     """A query stopped by its time limit reports E_TIMEOUT and a count that is a
     lower bound; every query finished within the limit is exact (plain and batch)."""
