@@ -26,7 +26,8 @@ at most 64) into one csm_batch_match; p50/p99 latency from arrival to result
 beside a FIFO plain per-query server over the same arrivals.
 
 cpu_baseline times the CPU oracle (same graph, queries, matching order and
-cap) on a bounded sample of the headline workload.
+cap) on a bounded sample of the headline workload, with up to 16 threads and
+single-threaded, and names the host CPU.
 """
 import argparse
 import json
@@ -109,21 +110,41 @@ def kernels_json(tot):
             for c, v in tot.items() if v[0]}
 
 
-def cpu_baseline(hg, host_queries, cap, seconds):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hg, host_queries, cap, seconds, single_seconds=6.0):
+    """The CPU backtracking matcher (same matching order, filter and cap) on
+    the same graph and queries, each query stopped by a time budget: with up
+    to 16 threads over root candidates, then single-threaded."""
     from oracle import oracle as O
 
     labels, edges = hg.arrays()
     data = O.Data.from_arrays(labels, edges)
+
+    def run(threads, budget):
+        total, secs, timed_out = 0, 0.0, False
+        per = budget / len(host_queries)
+        for hq in host_queries:
+            ql, qe = hq.arrays()
+            oq = O.Query.from_arrays(ql, qe)
+            c, capped, _, to, s = O.match(data, oq, oq.plan(), cap=cap, threads=threads, budget_s=per)
+            total += c
+            secs += s
+            timed_out |= to
+        return total, secs, timed_out, per
+
     threads = min(os.cpu_count() or 1, 16)
-    total, secs, timed_out = 0, 0.0, False
-    per = seconds / len(host_queries)
-    for hq in host_queries:
-        ql, qe = hq.arrays()
-        oq = O.Query.from_arrays(ql, qe)
-        c, capped, _, to, s = O.match(data, oq, oq.plan(), cap=cap, threads=threads, budget_s=per)
-        total += c
-        secs += s
-        timed_out |= to
+    total, secs, timed_out, per = run(threads, seconds)
+    t1, s1, to1, per1 = run(1, single_seconds)
     return {
         "value": round(total / secs, 1) if secs else 0.0,
         "unit": "embeddings/s",
@@ -132,13 +153,21 @@ def cpu_baseline(hg, host_queries, cap, seconds):
         "sample": f"same graph and queries, backtracking oracle with {threads} threads, "
                   f"{per:.0f} s budget per query ({'stopped by the budget' if timed_out else 'to the cap'}), "
                   f"{total} embeddings in {secs:.1f} s",
+        "single_thread": {"value": round(t1 / s1, 1) if s1 else 0.0, "unit": "embeddings/s",
+                          "sample": f"same queries, 1 thread, {per1:.0f} s budget per query "
+                                    f"({'stopped by the budget' if to1 else 'to the cap'}), {t1} embeddings in {s1:.1f} s"},
+        "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
     }
 
 
-def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0):
+def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0, sample=0):
     """The C3-shaped query set: the plain path query by query (each with a time
     limit), then one batch call over every query the plain path finished,
-    exact (no limit); queries/s compared on that same set."""
+    exact (no limit); queries/s compared on that same set.
+
+    sample > 0 is for counter profiles: only the batch, with default options,
+    over the first `sample` queries (warm-up call and timed call), with no
+    plain leg and no autotuner."""
     t0 = time.time()
     hg = csm.gen_rmat(20, 1 << 24, 32, 1.0, 3)
     g = ctx.upload(hg)
@@ -151,6 +180,21 @@ def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0):
         plans.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 1000 + i, avg_degree=avg).plan())
     qs = [g.prepare(p) for p in plans]
     print(f"# batched setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    if sample:
+        kq = qs[:sample]
+        csm.batch_match(g, kq, cap=cap)
+        ctx.set_profiling(True)
+        t2 = time.perf_counter()
+        res = csm.batch_match(g, kq, cap=cap)
+        t_batch = time.perf_counter() - t2
+        ctx.set_profiling(False)
+        print(f"# batched sample: {len(kq)} queries in {t_batch:.2f}s", file=sys.stderr, flush=True)
+        out = {"profile_sample": f"first {len(kq)} of the {nqueries} queries, default options, batch only",
+               "queries_per_s": round(len(kq) / t_batch, 2), "kernels": kernels_json(kernel_totals([[res[0]]]))}
+        for q in qs:
+            q.free()
+        g.free()
+        return out
     plain, plain_s = [], []
     for q in qs:
         t1 = time.perf_counter()
@@ -349,6 +393,8 @@ def main():
     ap.add_argument("--batched-limit-ms", type=int, default=2000)
     ap.add_argument("--batched-item-budget", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-sample", type=int, default=0,
+                    help="profiling: batch only over the first N queries (no plain leg, no autotuner)")
     ap.add_argument("--arena-gb", type=float, default=96.0)
     ap.add_argument("--served-queries", type=int, default=128)
     ap.add_argument("--served-qps", type=float, default=32.0)
@@ -416,7 +462,7 @@ def main():
     g.free()
     if not args.no_batched:
         line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_limit_ms,
-                                          args.batched_item_budget)
+                                          args.batched_item_budget, args.batched_sample)
     if not args.no_served:
         line["served"] = served_section(ctx, args.served_queries, args.served_qps, args.batched_cap,
                                         args.batched_limit_ms, 64, 64)

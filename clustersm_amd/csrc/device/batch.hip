@@ -937,7 +937,8 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     sm.acc_count += count_level ? tot_contrib : 0;
     sm.acc_items += tot_items;
     sm.acc_rows += rows_written;
-    if (sm.acc_count >= (Q.cap >> 8) + 1 || sm.acc_items >= (1ull << 22)) flush_acc(a, sm);
+    // held-back counts over all workgroups stay below a quarter of the cap
+    if (sm.acc_count >= Q.cap / (4ull * gridDim.x) + 1 || sm.acc_items >= (1ull << 22)) flush_acc(a, sm);
     n_items += tot_items;
     n_rows_out += rows_written;
     sm.phase[3] += __builtin_amdgcn_s_memrealtime() - tp0;

@@ -5,7 +5,11 @@
 // when the byte saturates or the label is beyond the table) goes to x's rank
 // in the slot's offset array; one exclusive scan over all slots at once turns
 // the bounds into list starts in cn_adj; one pass over the adjacency then
-// writes every list, sorted, and its exact length.
+// writes every list, sorted, and its exact length.  Candidates of degree at
+// least kHeavyDegree are left out of that pass (one wave walking a word whose
+// vertices hold hundreds of thousands of neighbours would run long after the
+// rest): the bound pass lists them, and one workgroup per (slot, vertex)
+// writes each of their lists.
 #include <algorithm>
 #include <vector>
 
@@ -26,6 +30,13 @@ struct SlotDesc {
   uint64_t ncand;   // |C(src)|
   uint32_t dst_label;
   uint32_t pad;
+};
+
+// A (slot, candidate) pair whose list is built by cn_heavy_kernel.
+struct HeavyPair {
+  uint32_t slot;   // index into the slot array
+  uint32_t x;
+  uint64_t entry;  // the candidate's entry in cn_off / cn_len
 };
 
 struct RankJob {
@@ -50,10 +61,14 @@ __device__ inline bool bit_of(const uint64_t *bits, uint32_t v) { return (bits[v
 
 // List bounds: per slot and x in C(s) (blockIdx.y = slot), the number of
 // x's neighbours labelled label(t), else deg(x); a zero closes each slot.
+// Candidates of degree >= heavy_deg are appended to `heavy` (at most
+// heavy_cap pairs; *heavy_n counts every one, so the host sees an overflow).
 __global__ __launch_bounds__(kThreads) void bound_kernel(const uint32_t *__restrict__ off,
                                                          const uint8_t *__restrict__ nlc, uint32_t nlc_labels,
                                                          uint32_t words, const SlotDesc *__restrict__ slots,
-                                                         uint64_t *__restrict__ cn_off) {
+                                                         uint64_t *__restrict__ cn_off, uint32_t slot0,
+                                                         uint32_t heavy_deg, HeavyPair *__restrict__ heavy,
+                                                         unsigned long long *heavy_n, uint64_t heavy_cap) {
   const SlotDesc sd = slots[blockIdx.y];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const uint64_t *xrow = sd.bits + static_cast<uint64_t>(sd.src) * words;
@@ -61,16 +76,31 @@ __global__ __launch_bounds__(kThreads) void bound_kernel(const uint32_t *__restr
   const uint64_t pref0 = pref[0];
   const bool table = nlc != nullptr && sd.dst_label < nlc_labels;
   if (blockIdx.x == 0 && threadIdx.x == 0) cn_off[sd.base + sd.ncand] = 0;
-  for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {
+  for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {  // wave-uniform
     const uint64_t xw = xrow[b];
-    if (!((xw >> lane) & 1ull)) continue;
+    if (xw == 0) continue;
+    const bool act = (xw >> lane) & 1ull;
     const uint32_t x = b * 64u + static_cast<uint32_t>(lane);
-    uint32_t bound = off[x + 1] - off[x];
-    if (table) {
-      const uint32_t c = nlc[static_cast<uint64_t>(x) * kNlcStride + sd.dst_label];
-      if (c < 255) bound = min(bound, c);
+    const uint64_t entry = sd.base + (pref[b] - pref0) + lanes_below(xw);
+    uint32_t deg = 0;
+    if (act) {
+      deg = off[x + 1] - off[x];
+      uint32_t bound = deg;
+      if (table) {
+        const uint32_t c = nlc[static_cast<uint64_t>(x) * kNlcStride + sd.dst_label];
+        if (c < 255) bound = min(bound, c);
+      }
+      cn_off[entry] = bound;
     }
-    cn_off[sd.base + (pref[b] - pref0) + lanes_below(xw)] = bound;
+    const bool hv = act && deg >= heavy_deg;
+    const uint64_t hm = ballot(hv);
+    if (hm) {  // one counter update per wave
+      unsigned long long at = 0;
+      if (lane == 0) at = atomicAdd(heavy_n, static_cast<unsigned long long>(__popcll(hm)));
+      at = __shfl(at, 0, kWave);
+      const uint64_t idx = at + lanes_below(hm);
+      if (hv && idx < heavy_cap) heavy[idx] = HeavyPair{slot0 + blockIdx.y, x, entry};
+    }
   }
 }
 
@@ -85,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void bound_kernel(const uint32_t *__restr
 __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                                                       uint32_t words, const SlotDesc *__restrict__ slots,
                                                       const uint64_t *__restrict__ cn_off, uint32_t *__restrict__ cn_len,
-                                                      uint32_t *__restrict__ cn_adj) {
+                                                      uint32_t *__restrict__ cn_adj, uint32_t heavy_deg) {
   __shared__ uint32_t s_pre[kWaves][kWave + 1];  // exclusive prefix of the lanes' degrees (0 for unset lanes)
   __shared__ uint32_t s_o0[kWaves][kWave];       // adjacency start per lane
   __shared__ uint32_t s_run[kWaves][kWave];      // hits so far per lane
@@ -106,9 +136,14 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
     const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
     const uint64_t slot = sd.base + (pref[b] - pref0) + lanes_below(xw);
     uint32_t o0 = 0, dv = 0;
+    bool hv = false;  // heavy: its list is cn_heavy_kernel's, its range empty here
     if (act) {
       o0 = off[v];
       dv = off[v + 1] - o0;
+      if (dv >= heavy_deg) {
+        hv = true;
+        dv = 0;
+      }
     }
     uint32_t incl = dv;
     for (int d = 1; d < kWave; d <<= 1) {
@@ -146,8 +181,51 @@ __global__ __launch_bounds__(kThreads) void cn_kernel(const uint32_t *__restrict
       if (hit) atomicAdd(&s_run[wave][owner], 1u);
       wave_sync();
     }
-    if (act) cn_len[slot] = s_run[wave][lane];
+    if (act && !hv) cn_len[slot] = s_run[wave][lane];
     wave_sync();
+  }
+}
+
+// One workgroup per heavy (slot, x) pair: x's adjacency 256 entries at a
+// time, hits compacted in order (wave ballots, then the waves' counts in
+// LDS), so the list is sorted like every other; then its length.
+__global__ __launch_bounds__(kThreads) void cn_heavy_kernel(const uint32_t *__restrict__ off,
+                                                            const uint32_t *__restrict__ adj, uint32_t words,
+                                                            const SlotDesc *__restrict__ slots,
+                                                            const HeavyPair *__restrict__ pairs, uint64_t npairs,
+                                                            const uint64_t *__restrict__ cn_off,
+                                                            uint32_t *__restrict__ cn_len, uint32_t *__restrict__ cn_adj) {
+  __shared__ uint32_t s_cnt[kWaves];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (uint64_t p = blockIdx.x; p < npairs; p += gridDim.x) {  // uniform over the workgroup
+    const HeavyPair hp = pairs[p];
+    const SlotDesc &sd = slots[hp.slot];
+    const uint64_t *yrow = sd.bits + static_cast<uint64_t>(sd.dst) * words;
+    const uint32_t o0 = off[hp.x], deg = off[hp.x + 1] - o0;
+    const uint64_t start = cn_off[hp.entry];
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < deg; base += kThreads) {
+      const uint32_t e = base + threadIdx.x;
+      uint32_t w = 0;
+      bool hit = false;
+      if (e < deg) {
+        w = adj[o0 + e];
+        hit = bit_of(yrow, w);
+      }
+      const uint64_t m = ballot(hit);
+      if (lane == 0) s_cnt[wave] = static_cast<uint32_t>(__popcll(m));
+      __syncthreads();
+      uint32_t before = 0, tot = 0;
+      for (int i = 0; i < kWaves; ++i) {
+        const uint32_t c = s_cnt[i];
+        before += i < wave ? c : 0u;
+        tot += c;
+      }
+      if (hit) cn_adj[start + run + before + lanes_below(m)] = w;
+      run += tot;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) cn_len[hp.entry] = run;
   }
 }
 
@@ -213,6 +291,11 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   RankJob *d_jobs = static_cast<RankJob *>(get(sizeof(RankJob) * nqs));
   if (!cn_off || !cn_len || !partials || !total_d || !d_slots || !d_jobs)
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate index)");
+  // heavy pairs: at most every heavy vertex in every slot; without room for
+  // that, every list goes through the per-word walk
+  const uint64_t heavy_cap = static_cast<uint64_t>(g->num_heavy) * slots.size();
+  HeavyPair *heavy = heavy_cap ? static_cast<HeavyPair *>(get(sizeof(HeavyPair) * heavy_cap)) : nullptr;
+  const uint32_t heavy_deg = heavy ? kHeavyDegree : ~0u;
   std::vector<RankJob> jobs(nqs);
   for (uint32_t i = 0; i < nqs; ++i) jobs[i] = {qs[i]->bits, out[i].rank_pref, qs[i]->desc.nq};
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_jobs, jobs.data(), sizeof(RankJob) * nqs, hipMemcpyHostToDevice, s));
@@ -229,23 +312,35 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   const uint32_t gx = std::max<uint32_t>(1, std::min<uint32_t>(std::max<uint32_t>(8, 1024 / nqs), ceil_div(words, kWaves)));
   uint64_t cand_words = 0;
   for (auto &sd : slots) cand_words += sd.ncand;
+  unsigned long long *heavy_n = reinterpret_cast<unsigned long long *>(total_d + 1);
+  CSM_HIP_TRY(ctx, hipMemsetAsync(heavy_n, 0, 8, s));
   for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {  // grid.y limit
     const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
     kt.begin(CSM_K_AUX);
-    bound_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->nlc, g->nlc_labels, words, d_slots + s0, cn_off);
+    bound_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->nlc, g->nlc_labels, words, d_slots + s0, cn_off,
+                                                   static_cast<uint32_t>(s0), heavy_deg, heavy, heavy_n, heavy_cap);
     kt.end(CSM_K_AUX, 8ull * words * ns + 9ull * cand_words * ns / slots.size());
   }
   scan_u64_exclusive(cn_off, entries, partials, total_d, s);
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, total_d, 8, hipMemcpyDeviceToHost, s));
+  CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, total_d, 16, hipMemcpyDeviceToHost, s));
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-  const uint64_t total = ctx->pinned[0];
+  const uint64_t total = ctx->pinned[0], npairs = ctx->pinned[1];
+  if (npairs > heavy_cap)
+    return set_error(ctx, CSM_E_DEVICE, "candidate index: more heavy candidates than heavy vertices x slots");
   uint32_t *cn_adj = static_cast<uint32_t *>(get(4 * std::max<uint64_t>(total, 1)));
   if (!cn_adj) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate lists)");
   for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {
     const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
     kt.begin(CSM_K_AUX);
-    cn_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, cn_len, cn_adj);
+    cn_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->adj, words, d_slots + s0, cn_off, cn_len, cn_adj,
+                                                heavy_deg);
     kt.end(CSM_K_AUX, 8ull * words * ns + 4ull * total * ns / slots.size());
+  }
+  if (npairs) {
+    kt.begin(CSM_K_AUX);
+    cn_heavy_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(npairs, 65536)), kThreads, 0, s>>>(
+        g->off, g->adj, words, d_slots, heavy, npairs, cn_off, cn_len, cn_adj);
+    kt.end(CSM_K_AUX, 16ull * npairs);
   }
   for (uint32_t i = 0; i < nqs; ++i) {
     out[i].cn_adj = cn_adj;

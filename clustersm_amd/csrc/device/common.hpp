@@ -78,11 +78,15 @@ struct csm_graph {
   // kNlcStride bytes, byte l = |{w in N(v) : label(w) = l}| for l < nlc_labels.
   uint8_t *nlc = nullptr;
   uint32_t nlc_labels = 0;
+  uint32_t num_heavy = 0;  // vertices of degree >= csm::kHeavyDegree
 };
 
 namespace csm {
 
 constexpr uint32_t kNlcStride = 64;  // labels covered by csm_graph::nlc
+// Candidate lists of vertices at least this degree are built by a whole
+// workgroup each, outside the per-word walk of the other candidates.
+constexpr uint32_t kHeavyDegree = 1024;
 
 // Per-query descriptor, passed to kernels by value.  Replaces the reference's
 // __constant__ globals (globals.cu:25-38) so several queries can be resident.

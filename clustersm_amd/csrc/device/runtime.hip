@@ -225,8 +225,11 @@ csm_status build_hubs(csm_ctx *ctx, csm_graph *g) {
   CSM_HIP_TRY(ctx, hipMemcpyAsync(off.data(), g->off, 4ull * (nv + 1ull), hipMemcpyDeviceToHost, s));
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
   std::vector<uint32_t> cand;
-  for (uint32_t v = 0; v < nv; ++v)
+  g->num_heavy = 0;
+  for (uint32_t v = 0; v < nv; ++v) {
     if (off[v + 1] - off[v] >= kHubMinDegree) cand.push_back(v);
+    if (off[v + 1] - off[v] >= kHeavyDegree) ++g->num_heavy;
+  }
   std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
     uint32_t da = off[a + 1] - off[a], db = off[b + 1] - off[b];
     return da != db ? da > db : a < b;

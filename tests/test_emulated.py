@@ -68,3 +68,51 @@ def test_device_code_on_cpu_matches_oracle(driver, tmp_path, seed):
         want = _oracle(dl, de, queries, cap)
         assert plain == want, (cap, plain, want)
         assert batch == want, (cap, batch, want)
+
+
+def test_heavy_candidate_lists_on_cpu(driver, tmp_path):
+    """Vertices of degree >= 1024 have their candidate lists built by a
+    workgroup each, outside the per-word walk: a graph with three such hubs
+    (label 0, the only vertices of that label) and queries through them give
+    the oracle's counts, uncapped and capped."""
+    r = random.Random(12)
+    n = 2600
+    dl = [1 + r.randrange(3) for _ in range(n)]
+    hubs = [0, 700, 1901]
+    for h in hubs:
+        dl[h] = 0
+    edges = set()
+    for h in hubs:
+        for v in r.sample(range(n), 1000 + r.randrange(600)):
+            if v != h:
+                edges.add((min(h, v), max(h, v)))
+    while len(edges) < 9000:
+        a, b = r.randrange(n), r.randrange(n)
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    de = sorted(edges)
+    queries = [
+        ([0, 1], [(0, 1)]),
+        ([0, 1, 2], [(0, 1), (1, 2)]),
+        ([2, 0, 3], [(0, 1), (1, 2), (0, 2)]),
+        ([1, 0, 2, 3], [(0, 1), (0, 2), (2, 3), (1, 3)]),
+    ]
+    dpath = tmp_path / "data.graph"
+    csm.HostGraph.from_edges(dl, de).write(str(dpath))
+    qpaths = []
+    for i, (ql, qe) in enumerate(queries):
+        p = tmp_path / f"q{i}.graph"
+        csm.HostGraph.from_edges(ql, qe).write(str(p))
+        qpaths.append(str(p))
+    env = dict(os.environ, EMU_THREADS="4", EMU_CUS="2", ASAN_OPTIONS="detect_leaks=0")
+    for cap in (0, 50):
+        out = subprocess.run([driver, str(dpath), str(cap), *qpaths], env=env, capture_output=True, text=True,
+                             timeout=900)
+        assert out.returncode == 0, out.stderr[-3000:]
+        assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
+        lines = out.stdout.split("\n")
+        plain = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("plain")]
+        batch = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("batch")]
+        want = _oracle(dl, de, queries, cap)
+        assert plain == want, (cap, plain, want)
+        assert batch == want, (cap, batch, want)
