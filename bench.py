@@ -20,10 +20,11 @@ same set and every count must equal the plain path's.
 
 Served: R-MAT scale 22 (2^26 edge samples), 32 labels Zipf(1.0), resident;
 128 random-walk queries (4..32 vertices, geometric with mean 10) arriving as
-a Poisson stream at 32 queries/s.  A batched server admits every request
-that arrived while the previous batch ran (footprint estimate within 64 GB,
-at most 64) into one csm_batch_match; p50/p99 latency from arrival to result
-beside a FIFO plain per-query server over the same arrivals.
+a Poisson stream at 32 queries/s (and, in other_loads, at 64 and 96).  A
+batched server admits every request that arrived while the previous batch
+ran (footprint estimate within 64 GB, at most 64) into one csm_batch_match;
+p50/p99 latency from arrival to result beside a FIFO plain per-query server
+over the same arrivals.
 
 cpu_baseline times the CPU oracle (same graph, queries, matching order and
 cap) on a bounded sample of the headline workload, with up to 16 threads and
@@ -270,7 +271,7 @@ def footprint(plan, label_count, n, avg_degree, words):
     return int(est) + 4096
 
 
-def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed=7):
+def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed=7, extra_qps=None):
     """Served workload: a Poisson stream of random-walk queries (4..32
     vertices, geometric with mean 10) against a resident R-MAT scale 22 graph.
 
@@ -282,7 +283,8 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
     end.  Plain server: the same arrivals served one query at a time, FIFO,
     with each query's measured plain-path service time (prepare + match).
     Queries the plain path cannot finish within limit_ms are screened out
-    first, and every served count must equal its plain count."""
+    first, and every served count must equal its plain count.  extra_qps
+    adds arrival streams at other rates over the same queries (other_loads)."""
     t0 = time.time()
     hg = csm.gen_rmat(22, 1 << 26, 32, 1.0, 11)
     g = ctx.upload(hg)
@@ -311,55 +313,69 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
     keep = [i for i in range(nqueries) if plain[i].status == 0]
     print(f"# served: plain path finishes {len(keep)} of {nqueries} within {limit_ms} ms, "
           f"{sum(service[i] for i in keep):.2f}s of service", file=sys.stderr, flush=True)
-    # one Poisson arrival stream over the kept queries
-    arr, t = [], 0.0
-    for _ in keep:
-        t += r.expovariate(qps)
-        arr.append(t)
-    # plain server: FIFO over measured service times
-    lat_plain, free_at = [], 0.0
-    for a, i in zip(arr, keep):
-        start = max(a, free_at)
-        free_at = start + service[i]
-        lat_plain.append(free_at - a)
-    # batched server, live
     wq = g.prepare(plans[keep[0]])
     csm.batch_match(g, [wq], cap=cap)  # warm-up (code objects)
     wq.free()
-    lat = [0.0] * len(keep)
-    pending, nxt, done, sizes, mism = [], 0, 0, [], []
     budget = budget_gb * (1 << 30)
-    ts = time.perf_counter()
-    while done < len(keep):
-        now = time.perf_counter() - ts
-        while nxt < len(keep) and arr[nxt] <= now:
-            pending.append(nxt)
-            nxt += 1
-        if not pending:
-            time.sleep(max(0.0, arr[nxt] - now))
-            continue
-        batch, used = [], 0
-        for j in pending:
-            est = footprint(plans[keep[j]], label_count, n, avg_degree, words)
-            if batch and (len(batch) >= max_batch or used + est > budget):
-                break
-            batch.append(j)
-            used += est
-        pending = pending[len(batch):]
-        qs = [g.prepare(plans[keep[j]]) for j in batch]
-        res = csm.batch_match(g, qs, cap=cap)
-        tend = time.perf_counter() - ts
-        for j, x, q in zip(batch, res, qs):
-            lat[j] = tend - arr[j]
-            p = plain[keep[j]]
-            if (x.status, x.count, x.capped) != (0, p.count, p.capped):
-                mism.append([keep[j], x.count, int(x.capped), x.status, x.path, p.count, int(p.capped), len(batch)])
-            q.free()
-        sizes.append(len(batch))
-        done += len(batch)
-    span = time.perf_counter() - ts
-    g.free()
     pct = lambda v, f: round(1e3 * float(np.percentile(v, f)), 1)
+    mism = []
+
+    def run_load(rate, rs):
+        # one Poisson arrival stream over the kept queries
+        arr, t = [], 0.0
+        for _ in keep:
+            t += rs.expovariate(rate)
+            arr.append(t)
+        # plain server: FIFO over measured service times
+        lat_plain, free_at = [], 0.0
+        for a, i in zip(arr, keep):
+            start = max(a, free_at)
+            free_at = start + service[i]
+            lat_plain.append(free_at - a)
+        # batched server, live
+        lat = [0.0] * len(keep)
+        pending, nxt, done, sizes = [], 0, 0, []
+        ts = time.perf_counter()
+        while done < len(keep):
+            now = time.perf_counter() - ts
+            while nxt < len(keep) and arr[nxt] <= now:
+                pending.append(nxt)
+                nxt += 1
+            if not pending:
+                time.sleep(max(0.0, arr[nxt] - now))
+                continue
+            batch, used = [], 0
+            for j in pending:
+                est = footprint(plans[keep[j]], label_count, n, avg_degree, words)
+                if batch and (len(batch) >= max_batch or used + est > budget):
+                    break
+                batch.append(j)
+                used += est
+            pending = pending[len(batch):]
+            qs = [g.prepare(plans[keep[j]]) for j in batch]
+            res = csm.batch_match(g, qs, cap=cap)
+            tend = time.perf_counter() - ts
+            for j, x, q in zip(batch, res, qs):
+                lat[j] = tend - arr[j]
+                p = plain[keep[j]]
+                if (x.status, x.count, x.capped) != (0, p.count, p.capped):
+                    mism.append([keep[j], x.count, int(x.capped), x.status, x.path, p.count, int(p.capped),
+                                 len(batch), rate])
+                q.free()
+            sizes.append(len(batch))
+            done += len(batch)
+        span = time.perf_counter() - ts
+        print(f"# served at {rate} queries/s: {len(sizes)} batches in {span:.2f}s", file=sys.stderr, flush=True)
+        return {"offered_qps": rate,
+                "batched": {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "queries_per_s": round(len(keep) / span, 2),
+                            "batches": len(sizes), "mean_batch": round(sum(sizes) / max(1, len(sizes)), 1)},
+                "plain_fifo": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
+                               "queries_per_s": round(len(keep) / max(free_at - arr[0], 1e-9), 2),
+                               "mean_service_ms": round(1e3 * sum(service[i] for i in keep) / max(1, len(keep)), 1)}}
+
+    main = run_load(qps, r)
+    more = [run_load(x, random.Random(seed + 1 + k)) for k, x in enumerate(extra_qps or [])]
+    g.free()
     return {
         "workload": f"R-MAT scale 22 (2^26 samples, 32 Zipf(1.0) labels) resident; Poisson arrivals at {qps} "
                     f"queries/s of random-walk queries, 4..32 vertices (geometric, mean 10), average degree 2..4, "
@@ -370,11 +386,9 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
         "queries": len(keep),
         "screened_out": [i for i in range(nqueries) if i not in keep],
         "offered_qps": qps,
-        "batched": {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "queries_per_s": round(len(keep) / span, 2),
-                    "batches": len(sizes), "mean_batch": round(sum(sizes) / max(1, len(sizes)), 1)},
-        "plain_fifo": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
-                       "queries_per_s": round(len(keep) / max(free_at - arr[0], 1e-9), 2),
-                       "mean_service_ms": round(1e3 * sum(service[i] for i in keep) / max(1, len(keep)), 1)},
+        "batched": main["batched"],
+        "plain_fifo": main["plain_fifo"],
+        "other_loads": more,
         "counts_equal_plain": not mism,
         "mismatches": mism,
     }
@@ -398,6 +412,8 @@ def main():
     ap.add_argument("--arena-gb", type=float, default=96.0)
     ap.add_argument("--served-queries", type=int, default=128)
     ap.add_argument("--served-qps", type=float, default=32.0)
+    ap.add_argument("--served-more-qps", type=str, default="64,96",
+                    help="comma-separated further arrival rates for the served workload ('' for none)")
     ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -464,8 +480,9 @@ def main():
         line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_limit_ms,
                                           args.batched_item_budget, args.batched_sample)
     if not args.no_served:
+        more = [float(x) for x in args.served_more_qps.split(",") if x.strip()]
         line["served"] = served_section(ctx, args.served_queries, args.served_qps, args.batched_cap,
-                                        args.batched_limit_ms, 64, 64)
+                                        args.batched_limit_ms, 64, 64, extra_qps=more)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hg, [hq for _, hq in named], args.cap, args.cpu_seconds)
     ctx.close()

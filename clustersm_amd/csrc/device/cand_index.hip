@@ -291,11 +291,13 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   RankJob *d_jobs = static_cast<RankJob *>(get(sizeof(RankJob) * nqs));
   if (!cn_off || !cn_len || !partials || !total_d || !d_slots || !d_jobs)
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate index)");
-  // heavy pairs: at most every heavy vertex in every slot; without room for
-  // that, every list goes through the per-word walk
-  const uint64_t heavy_cap = static_cast<uint64_t>(g->num_heavy) * slots.size();
+  // heavy pairs: room for every heavy vertex in every slot, within 1/64 of
+  // the free arena; when the pairs do not fit, every list goes through the
+  // per-word walk (the bounds are the same either way)
+  uint64_t heavy_cap = std::min<uint64_t>(static_cast<uint64_t>(g->num_heavy) * slots.size(),
+                                          std::max<uint64_t>(1ull << 16, ctx->arena.largest_free() / 64 / sizeof(HeavyPair)));
   HeavyPair *heavy = heavy_cap ? static_cast<HeavyPair *>(get(sizeof(HeavyPair) * heavy_cap)) : nullptr;
-  const uint32_t heavy_deg = heavy ? kHeavyDegree : ~0u;
+  if (!heavy) heavy_cap = 0;
   std::vector<RankJob> jobs(nqs);
   for (uint32_t i = 0; i < nqs; ++i) jobs[i] = {qs[i]->bits, out[i].rank_pref, qs[i]->desc.nq};
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_jobs, jobs.data(), sizeof(RankJob) * nqs, hipMemcpyHostToDevice, s));
@@ -318,15 +320,17 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
     const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
     kt.begin(CSM_K_AUX);
     bound_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(g->off, g->nlc, g->nlc_labels, words, d_slots + s0, cn_off,
-                                                   static_cast<uint32_t>(s0), heavy_deg, heavy, heavy_n, heavy_cap);
+                                                   static_cast<uint32_t>(s0), heavy ? kHeavyDegree : ~0u, heavy,
+                                                   heavy_n, heavy_cap);
     kt.end(CSM_K_AUX, 8ull * words * ns + 9ull * cand_words * ns / slots.size());
   }
   scan_u64_exclusive(cn_off, entries, partials, total_d, s);
   CSM_HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, total_d, 16, hipMemcpyDeviceToHost, s));
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-  const uint64_t total = ctx->pinned[0], npairs = ctx->pinned[1];
-  if (npairs > heavy_cap)
-    return set_error(ctx, CSM_E_DEVICE, "candidate index: more heavy candidates than heavy vertices x slots");
+  const uint64_t total = ctx->pinned[0];
+  const bool use_heavy = heavy && ctx->pinned[1] <= heavy_cap;
+  const uint64_t npairs = use_heavy ? ctx->pinned[1] : 0;
+  const uint32_t heavy_deg = use_heavy ? kHeavyDegree : ~0u;
   uint32_t *cn_adj = static_cast<uint32_t *>(get(4 * std::max<uint64_t>(total, 1)));
   if (!cn_adj) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (candidate lists)");
   for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {
