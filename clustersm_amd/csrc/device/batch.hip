@@ -303,8 +303,10 @@ struct Smem {
   LevelJoin lv_last;
   uint32_t roots[kMaxRootWords * 64];
   uint64_t pre[kRowBatch + 1];
-  uint64_t abeg[kRowBatch][kMaxA];  // candidate list of each kept anchor: begin
-  uint32_t alen[kRowBatch][kMaxA];  // and length
+  // candidate list of each kept anchor, anchor-major so that lanes reading
+  // one anchor of different rows hit consecutive banks: begin
+  uint64_t abeg[kMaxA][kRowBatch];
+  uint32_t alen[kMaxA][kRowBatch];  // and length
   uint64_t dbeg[kRowBatch];         // driver list start (after a resume skip)
   // fused count, when the last position's parent is an earlier column: per
   // row, |N(x) ∩ C(u_last)| minus the row's own members of that list, and the list
@@ -653,8 +655,8 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       const uint32_t r = p / nkept, x = p - r * nkept;
       uint64_t b0, e0;
       Q.ci.list(sm.lv, x, s_rows[r * k + sm.lv.anchor[x]], b0, e0);
-      sm.abeg[r][x] = b0;
-      sm.alen[r][x] = static_cast<uint32_t>(e0 - b0);
+      sm.abeg[x][r] = b0;
+      sm.alen[x][r] = static_cast<uint32_t>(e0 - b0);
     }
     __syncthreads();
     // each row: the shortest list drives
@@ -664,9 +666,9 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       uint64_t best = ~0ull, bbeg = 0;
       uint32_t bi = 0;
       for (uint32_t x = 0; x < nkept; ++x) {
-        if (sm.alen[tid][x] < best) {
-          best = sm.alen[tid][x];
-          bbeg = sm.abeg[tid][x];
+        if (sm.alen[x][tid] < best) {
+          best = sm.alen[x][tid];
+          bbeg = sm.abeg[x][tid];
           bi = x;
         }
       }
@@ -757,8 +759,8 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j) {
             const bool act = ok[j] && sm.drv[ri[j]] != x;
-            lo[j] = sm.abeg[ri[j]][x];
-            n[j] = act ? sm.alen[ri[j]][x] : 0u;
+            lo[j] = sm.abeg[x][ri[j]];
+            n[j] = act ? sm.alen[x][ri[j]] : 0u;
           }
           lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f);
 #pragma unroll
