@@ -251,6 +251,11 @@ def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0, sample=0):
         "kernels": kernels_json(tot),
         "join_kernel": {"ms": round(bk[1] / 1e6, 1), "algorithmic_GB": round(bk[2] / 1e9, 2),
                         "GB_per_s": round(bk[2] / bk[1], 1) if bk[1] else None},
+        # candidate tests (row x driver entry) done by each path; capped
+        # queries overshoot their cap by different amounts on the two paths
+        "items": {"batch": sum(x.items for x in res), "plain": sum(plain[i].items for i in keep),
+                  "top_batch": [[i, x.items, plain[i].items, int(x.capped)]
+                                for x, i in sorted(zip(res, keep), key=lambda t: -t[0].items)[:8]]},
     }
     for q in qs:
         q.free()
