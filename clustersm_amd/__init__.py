@@ -143,6 +143,8 @@ SIGNATURES = {
     "csm_host_graph_info": ([_vp, _u32p, _u64p], C.c_int),
     "csm_host_graph_views": ([_vp, C.POINTER(_u32p), C.POINTER(_u32p)], C.c_int),
     "csm_host_graph_write": ([_vp, C.c_char_p], C.c_int),
+    "csm_host_graph_write_bin": ([_vp, C.c_char_p], C.c_int),
+    "csm_host_graph_read_bin": ([C.c_char_p, C.POINTER(_vp)], C.c_int),
     "csm_host_last_error": ([], C.c_char_p),
     "csm_plan_build": ([_vp, C.POINTER(QueryPlan)], C.c_int),
     "csm_open": ([C.c_int, C.c_uint64, C.POINTER(_vp)], C.c_int),
@@ -231,6 +233,12 @@ class HostGraph:
         return cls(h)
 
     @classmethod
+    def read_bin(cls, path):
+        h = _vp()
+        _host_check(lib().csm_host_graph_read_bin(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def from_edges(cls, labels, edges):
         labels, lp = _u32(labels)
         edges, ep = _u32(np.asarray(edges, dtype=np.uint32).reshape(-1))
@@ -261,6 +269,9 @@ class HostGraph:
 
     def write(self, path):
         _host_check(lib().csm_host_graph_write(self.h, os.fsencode(path)))
+
+    def write_bin(self, path):
+        _host_check(lib().csm_host_graph_write_bin(self.h, os.fsencode(path)))
 
     def plan(self):
         p = QueryPlan()

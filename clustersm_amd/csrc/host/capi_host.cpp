@@ -86,6 +86,16 @@ csm_status csm_host_graph_write(const csm_host_graph *g, const char *path) {
   return csm::write_graph_file(*g, path);
 }
 
+csm_status csm_host_graph_write_bin(const csm_host_graph *g, const char *path) {
+  if (!g || !path) return CSM_E_ARG;
+  return csm::write_graph_bin(*g, path);
+}
+
+csm_status csm_host_graph_read_bin(const char *path, csm_host_graph **out) {
+  if (!path) return CSM_E_ARG;
+  return make_graph(out, [&](csm_host_graph &g) { return csm::read_graph_bin(path, g, &g_host_error); });
+}
+
 csm_status csm_plan_build(const csm_host_graph *query, csm_query_plan *out) {
   if (!query || !out) return CSM_E_ARG;
   return csm::build_plan(*query, *out, &g_host_error);

@@ -34,6 +34,11 @@ csm_status graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t m,
 
 csm_status write_graph_file(const csm_host_graph &g, const std::string &path);
 
+// Binary form of the same graph (labels and file-order edge list), for
+// graphs too large to re-parse as text on every run.
+csm_status write_graph_bin(const csm_host_graph &g, const std::string &path);
+csm_status read_graph_bin(const std::string &path, csm_host_graph &g, std::string *err);
+
 // Builds g.off / g.adj (threaded for large graphs).
 void build_sorted_csr(csm_host_graph &g);
 

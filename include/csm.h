@@ -70,6 +70,13 @@ csm_status csm_host_graph_views(const csm_host_graph *g, const uint32_t **labels
                                 const uint32_t **edges);
 /* Write in the t/v/e format (declared degree = number of edge lines). */
 csm_status csm_host_graph_write(const csm_host_graph *g, const char *path);
+/* Binary graph file: labels and the file-order edge list as raw u32 arrays
+ * behind a 24-byte header ("CSMGBIN1", u32 N, u32 0, u64 M).  No counterpart
+ * in the reference, whose only input is the text format (io.cu:24-108);
+ * for graphs too large to re-parse as text on every run.  Reading checks
+ * the header, the file size and every endpoint < N (CSM_E_FORMAT). */
+csm_status csm_host_graph_write_bin(const csm_host_graph *g, const char *path);
+csm_status csm_host_graph_read_bin(const char *path, csm_host_graph **out);
 /* Message of the last parser / planner failure on the calling thread. */
 const char *csm_host_last_error(void);
 
