@@ -36,7 +36,7 @@ void print_help() {
       "Options:\n"
       "  -h,--help                   Print this help message and exit\n"
       "  -q TEXT REQUIRED            query graph path\n"
-      "  -d TEXT REQUIRED            data graph path\n"
+      "  -d TEXT REQUIRED            data graph path (t/v/e text, or a binary graph file)\n"
       "  --gpu UINT                  gpu number\n"
       "  --cap UINT                  result cap (0 = none)\n"
       "  --list                      print the embeddings\n"
@@ -128,7 +128,14 @@ int main(int argc, char **argv) {
   }
 
   csm_host_graph *hg = nullptr;
-  csm_status st = csm_host_graph_read(dpath.c_str(), &hg);
+  // a binary graph file (csm_host_graph_write_bin) is recognised by its magic
+  bool bin = false;
+  if (FILE *f = std::fopen(dpath.c_str(), "rb")) {
+    char magic[8] = {};
+    bin = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "CSMGBIN1", 8) == 0;
+    std::fclose(f);
+  }
+  csm_status st = bin ? csm_host_graph_read_bin(dpath.c_str(), &hg) : csm_host_graph_read(dpath.c_str(), &hg);
   if (st != CSM_OK) die(st, dpath.c_str(), csm_host_last_error());
   std::vector<std::string> qpaths;
   if (!batch_path.empty()) {
