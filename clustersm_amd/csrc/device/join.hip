@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <numeric>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "cand_index.hpp"
 
@@ -375,6 +376,50 @@ enum : int {
   kCtlSize = 8
 };
 
+// List mode.  Query vertex of each matching-order position, by value.
+struct ColMap {
+  uint32_t u[CSM_MAX_QV];
+};
+
+// n rows of nq ids in matching order -> query-id order, written at dst.
+__global__ void to_query_order_kernel(const uint32_t *src, uint64_t n, uint32_t nq, ColMap cm, uint32_t *dst) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n * nq; i += 256ull * gridDim.x) {
+    const uint64_t r = i / nq;
+    const uint32_t p = static_cast<uint32_t>(i - r * nq);
+    dst[r * nq + cm.u[p]] = src[i];
+  }
+}
+
+__global__ void iota_kernel(uint32_t *p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) p[i] = static_cast<uint32_t>(i);
+}
+
+// keys[i] = column c of row perm[i]
+__global__ void gather_col_kernel(const uint32_t *rows, uint32_t nq, uint32_t c, const uint32_t *perm, uint64_t n,
+                                  uint32_t *keys) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x)
+    keys[i] = rows[static_cast<uint64_t>(perm[i]) * nq + c];
+}
+
+// out row i = row perm[i]
+__global__ void gather_rows_kernel(const uint32_t *rows, uint32_t nq, const uint32_t *perm, uint64_t n, uint32_t *out) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n * nq; i += 256ull * gridDim.x) {
+    const uint64_t r = i / nq;
+    out[i] = rows[static_cast<uint64_t>(perm[r]) * nq + (i - r * nq)];
+  }
+}
+
+// Sorts nl rows of nq ids (query-id order) lexicographically on the host.
+void sort_rows_host(std::vector<uint32_t> &qrows, uint64_t nl, uint32_t nq, uint32_t *out) {
+  std::vector<uint64_t> idx(nl);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](uint64_t x, uint64_t y) {
+    return std::lexicographical_compare(qrows.begin() + x * nq, qrows.begin() + (x + 1) * nq,
+                                        qrows.begin() + y * nq, qrows.begin() + (y + 1) * nq);
+  });
+  for (uint64_t i = 0; i < nl; ++i) std::copy(qrows.begin() + idx[i] * nq, qrows.begin() + (idx[i] + 1) * nq, out + i * nq);
+}
+
 struct Workspace {
   csm_ctx *ctx = nullptr;
   std::vector<void *> blocks;
@@ -484,6 +529,14 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     okalloc = ws.fbase && ws.flen && ws.fbeg && ws.row_sums;
   }
   if (!okalloc) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (join workspace)");
+  // list mode: the list stays on the device in query-id order and is sorted
+  // there when the arena holds it; otherwise each chunk is copied out and the
+  // list is sorted on the host
+  const uint64_t list_lim = want_list ? std::min(cap, o.list_capacity) : 0;
+  uint32_t *dlist = nullptr;
+  if (list_lim && list_lim < (1ull << 32)) dlist = static_cast<uint32_t *>(ws.get(4ull * nq * list_lim));
+  ColMap cmap{};
+  for (uint32_t p = 0; p < nq; ++p) cmap.u[p] = lv[p].u;
   CSM_HIP_TRY(ctx, hipMemsetAsync(ws.ctl, 0, 8 * kCtlSize, s));
   uint64_t *pin = ctx->pinned;
   if (rows2) {  // two vertices: the count is the total length of the one slot's lists
@@ -665,7 +718,9 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     if (k == L) {  // list mode: rows are complete embeddings in lexicographic order
       const uint64_t lim = std::min(cap, o.list_capacity);
       const uint64_t want = total < lim ? std::min(produced, lim - total) : 0;
-      if (want) {
+      if (want && dlist) {  // appended after the `total` rows listed so far
+        to_query_order_kernel<<<grid_for(want * nq), 256, 0, s>>>(ws.rows[nq], want, nq, cmap, dlist + total * nq);
+      } else if (want) {
         const size_t before = host_rows.size();
         host_rows.resize(before + want * nq);
         CSM_HIP_TRY(ctx, hipMemcpyAsync(host_rows.data() + before, ws.rows[nq], 4ull * nq * want,
@@ -691,20 +746,50 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   kt.flush();
   res.count = std::min(total, cap);
   res.capped = total > cap;
-  if (want_list) {
+  if (want_list && dlist) {
+    // stable LSD radix sort over the columns, last to first, carrying a row
+    // permutation; then one gather and one copy out
+    const uint64_t nl = std::min(total, list_lim);
+    res.listed = nl;
+    if (nl) {
+      uint32_t *k0 = static_cast<uint32_t *>(ws.get(4 * nl)), *k1 = static_cast<uint32_t *>(ws.get(4 * nl));
+      uint32_t *p0 = static_cast<uint32_t *>(ws.get(4 * nl)), *p1 = static_cast<uint32_t *>(ws.get(4 * nl));
+      uint32_t *out = static_cast<uint32_t *>(ws.get(4ull * nl * nq));
+      size_t tmp_bytes = 0;
+      void *tmp = nullptr;
+      if (k0 && k1 && p0 && p1 && out) {
+        rocprim::double_buffer<uint32_t> kd(k0, k1), pd(p0, p1);
+        CSM_HIP_TRY(ctx, rocprim::radix_sort_pairs(nullptr, tmp_bytes, kd, pd, nl, 0, 32, s));
+        tmp = ws.get(std::max<size_t>(tmp_bytes, 1));
+      }
+      if (tmp) {
+        kt.begin(CSM_K_AUX);
+        iota_kernel<<<grid_for(nl), 256, 0, s>>>(p0, nl);
+        uint32_t *perm = p0;
+        for (int c = static_cast<int>(nq) - 1; c >= 0; --c) {
+          gather_col_kernel<<<grid_for(nl), 256, 0, s>>>(dlist, nq, static_cast<uint32_t>(c), perm, nl, k0);
+          rocprim::double_buffer<uint32_t> kd(k0, k1), pd(perm, perm == p0 ? p1 : p0);
+          CSM_HIP_TRY(ctx, rocprim::radix_sort_pairs(tmp, tmp_bytes, kd, pd, nl, 0, 32, s));
+          perm = pd.current();
+        }
+        gather_rows_kernel<<<grid_for(nl * nq), 256, 0, s>>>(dlist, nq, perm, nl, out);
+        kt.end(CSM_K_AUX, 24ull * nl * nq + 8ull * nl * nq);
+        CSM_HIP_TRY(ctx, hipMemcpyAsync(o.list, out, 4ull * nl * nq, hipMemcpyDeviceToHost, s));
+        CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+      } else {  // no room for the sort: sort the copy on the host
+        std::vector<uint32_t> qrows(nl * nq);
+        CSM_HIP_TRY(ctx, hipMemcpyAsync(qrows.data(), dlist, 4ull * nl * nq, hipMemcpyDeviceToHost, s));
+        CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+        sort_rows_host(qrows, nl, nq, o.list);
+      }
+    }
+  } else if (want_list) {
     // matching-order rows -> query-id order, then sort lexicographically
     const uint64_t nl = host_rows.size() / nq;
     std::vector<uint32_t> qrows(host_rows.size());
     for (uint64_t i = 0; i < nl; ++i)
       for (uint32_t p = 0; p < nq; ++p) qrows[i * nq + lv[p].u] = host_rows[i * nq + p];
-    std::vector<uint64_t> idx(nl);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](uint64_t x, uint64_t y) {
-      return std::lexicographical_compare(qrows.begin() + x * nq, qrows.begin() + (x + 1) * nq,
-                                          qrows.begin() + y * nq, qrows.begin() + (y + 1) * nq);
-    });
-    for (uint64_t i = 0; i < nl; ++i)
-      std::copy(qrows.begin() + idx[i] * nq, qrows.begin() + (idx[i] + 1) * nq, o.list + i * nq);
+    sort_rows_host(qrows, nl, nq, o.list);
     res.listed = nl;
   }
   finish_time();
