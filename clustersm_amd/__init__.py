@@ -164,6 +164,7 @@ SIGNATURES = {
     "csm_query_free": ([_vp, _vp], C.c_int),
     "csm_filter": ([_vp, _vp, _vp, C.c_int, _u64p], C.c_int),
     "csm_candidates_download": ([_vp, _vp, _u64p], C.c_int),
+    "csm_query_reorder": ([_vp, _vp, _vp, C.c_int], C.c_int),
     "csm_match": ([_vp, _vp, _vp, C.POINTER(MatchOpts), C.POINTER(Result)], C.c_int),
     "csm_batch_match": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.POINTER(Result)], C.c_int),
     "csm_batch_autotune": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.c_uint32,
@@ -443,6 +444,11 @@ class Query:
         bits = np.zeros(self.plan.num_v * words, np.uint64)
         self.graph.ctx.check(lib().csm_candidates_download(self.graph.ctx.h, self.h, bits.ctypes.data_as(_u64p)))
         return bits.reshape(self.plan.num_v, words)
+
+    def reorder(self, refine=True):
+        """Candidate-aware matching order (csm_query_reorder); counts are unchanged."""
+        self.graph.ctx.check(lib().csm_query_reorder(self.graph.ctx.h, self.graph.h, self.h, 1 if refine else 0),
+                             "csm_query_reorder")
 
     def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0,
               time_limit_ms=0):

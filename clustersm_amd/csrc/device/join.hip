@@ -796,9 +796,84 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   return CSM_OK;
 }
 
+// Candidate-aware matching order: the root has the fewest candidates per
+// query degree; each next position takes the unmatched vertex with the most
+// matched neighbours, then the fewest candidates, then the highest degree,
+// then the lowest id.  Its parent is its earliest matched neighbour, the
+// others are its non-tree (back) positions.
+csm_status plan_reorder_by_candidates(csm_query_plan &plan, const uint64_t *counts) {
+  const uint32_t nq = plan.num_v;
+  bool adj[CSM_MAX_QV][CSM_MAX_QV] = {};
+  for (uint32_t e = 0; e < plan.num_e; ++e) {
+    const uint32_t a = plan.edge_src[e], b = plan.edge_dst[e];
+    if (a >= nq || b >= nq) return CSM_E_PLAN;
+    if (a != b) adj[a][b] = adj[b][a] = true;
+  }
+  uint32_t order[CSM_MAX_QV], pos[CSM_MAX_QV];
+  for (uint32_t u = 0; u < nq; ++u) pos[u] = CSM_NO_PARENT;
+  uint32_t root = 0;
+  for (uint32_t u = 1; u < nq; ++u)  // counts[u] / (deg[u] + 1) < counts[root] / (deg[root] + 1)
+    if (static_cast<double>(counts[u]) * (plan.degree[root] + 1) <
+        static_cast<double>(counts[root]) * (plan.degree[u] + 1))
+      root = u;
+  order[0] = root;
+  pos[root] = 0;
+  for (uint32_t p = 1; p < nq; ++p) {
+    uint32_t best = CSM_NO_PARENT, best_nb = 0;
+    for (uint32_t v = 0; v < nq; ++v) {
+      if (pos[v] != CSM_NO_PARENT) continue;
+      uint32_t nb = 0;
+      for (uint32_t w = 0; w < nq; ++w) nb += adj[v][w] && pos[w] != CSM_NO_PARENT;
+      if (nb == 0) continue;
+      const bool better = best == CSM_NO_PARENT || nb > best_nb ||
+                          (nb == best_nb && (counts[v] < counts[best] ||
+                                             (counts[v] == counts[best] && plan.degree[v] > plan.degree[best])));
+      if (better) {
+        best = v;
+        best_nb = nb;
+      }
+    }
+    if (best == CSM_NO_PARENT) return CSM_E_QUERY;  // not connected
+    order[p] = best;
+    pos[best] = p;
+  }
+  for (uint32_t p = 0; p < nq; ++p) {
+    const uint32_t u = order[p];
+    plan.match_vertex[p] = u;
+    plan.match_parent[p] = CSM_NO_PARENT;
+    plan.num_back[p] = 0;
+    if (p == 0) continue;
+    for (uint32_t j = 0; j < p; ++j) {  // earlier positions in order
+      if (!adj[u][order[j]]) continue;
+      if (plan.match_parent[p] == CSM_NO_PARENT) plan.match_parent[p] = j;
+      else plan.back_pos[p][plan.num_back[p]++] = j;
+    }
+  }
+  return CSM_OK;
+}
+
 }  // namespace csm
 
 using namespace csm;
+
+extern "C" csm_status csm_query_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine) {
+  if (!ctx || !g || !q || q->graph_n != g->n) return CSM_E_ARG;
+  hipSetDevice(ctx->device);
+  if (!q->filtered) {
+    const csm_status st = run_filter(ctx, g, q, refine);
+    if (st != CSM_OK) return st;
+  }
+  csm_query_plan plan = q->plan;
+  csm_status st = plan_reorder_by_candidates(plan, q->counts);
+  if (st != CSM_OK) return set_error(ctx, st, "query is not connected");
+  QueryDesc d;
+  LevelDesc lv[CSM_MAX_QV];
+  st = query_build_desc(plan, g->n, d, lv);
+  if (st != CSM_OK) return set_error(ctx, st, "reordered plan is invalid");
+  q->plan = plan;
+  std::memcpy(q->level, lv, sizeof(lv));
+  return CSM_OK;
+}
 
 extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts *opts,
                                 csm_result *out) {

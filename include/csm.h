@@ -242,6 +242,17 @@ csm_status csm_copy_probe(csm_ctx *ctx, uint64_t bytes, uint32_t iters, double *
 csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q,
                      const csm_match_opts *opts, csm_result *out);
 
+/* Candidate-aware matching order for q, from its filtered candidate counts
+ * (the filter runs first if it has not): the root has the fewest candidates
+ * per query degree, then each position takes the unmatched vertex with the
+ * most matched neighbours, the fewest candidates, the highest degree.  No
+ * counterpart in the reference, which always matches in the BFS edge order
+ * from the last cluster's core (main.cu:144-146, order.cpp:150-205; its
+ * candidate-count choice is commented out at order.cpp:160-173).  Counts are
+ * unchanged; a capped list then holds the first embeddings of the new order,
+ * which is why the default order stays the reference's. */
+csm_status csm_query_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine);
+
 /* ----------------------------------------------------------------- batched
  * No reference counterpart (the reference runs one process per query,
  * runClusterSM.sh:9-13).  All queries run in one persistent kernel fed by a

@@ -283,3 +283,41 @@ def test_hub_probes_and_fused_counts(ctx):
         q.free()
     assert seen_fused and seen_back
     g.free()
+
+
+def test_reorder_keeps_counts_vs_oracle(ctx):
+    """csm_query_reorder changes the matching order only: counts and cap
+    flags stay the oracle's, on random graphs and queries with non-tree
+    edges, and equal the default order's on R-MAT queries."""
+    r = random.Random(77)
+    for round_ in range(6):
+        n = r.randint(40, 140)
+        nl = r.randint(1, 3)
+        dl, de = rand_graph(r, n, r.uniform(0.04, 0.12), nl)
+        g = ctx.upload_edges(dl, de)
+        d = O.Data.from_arrays(dl, de)
+        for _ in range(6):
+            ql, qe = rand_query(r, r.randint(3, 7), r.randint(0, 3), nl)
+            oq = O.Query.from_arrays(ql, qe)
+            q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+            q.reorder()
+            for cap in (0, 50):
+                want, wcap, _, _, _ = O.match(d, oq, oq.plan(), cap=cap if cap else (1 << 64) - 1, threads=4)
+                res, _ = q.match(cap=cap)
+                assert (res.count, bool(res.capped)) == (want, bool(wcap)), (round_, ql, qe, cap)
+            q.free()
+        g.free()
+    hg = csm.gen_rmat(12, 40000, 8, 1.0, 9)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    for i in range(12):
+        plan = csm.gen_query(host, 4 + i, csm.QUERY_EXTRA, 400 + i, extra_edges=i % 4).plan()
+        a, b = g.prepare(plan), g.prepare(plan)
+        b.reorder()
+        ra, _ = a.match(cap=2_000_000)
+        rb, _ = b.match(cap=2_000_000)
+        print("reorder", i, ra.count, rb.count, f"{ra.ms_total:.1f} ms -> {rb.ms_total:.1f} ms", flush=True)
+        assert (ra.count, ra.capped) == (rb.count, rb.capped), i
+        a.free()
+        b.free()
+    g.free()
