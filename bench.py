@@ -9,6 +9,8 @@ queries have more than 1e10 embeddings, so each is enumerated up to a
 per-query cap (count-only); value = embeddings counted / second.  Kernel
 times come from HIP events recorded on the launch stream during the timed
 steps; the roofline entry is the kernel class with the most device time.
+`reordered` times the same steps with the candidate-aware matching order
+(csm_query_reorder), which keeps every count.
 
 Batched: R-MAT scale 20 (2^24 edge samples), 32 labels Zipf(1.0); 256
 random-walk queries of 4..32 vertices with average degree 2..4, per-query
@@ -512,6 +514,23 @@ def main():
         "copy_probe_GBps": round(copy_gbs, 1),
     }
     for q in queries:
+        q.free()
+    # the same steps with the candidate-aware matching order (same counts)
+    rq = [g.prepare(hq.plan()) for _, hq in named]
+    for q in rq:
+        q.reorder()
+    for _ in range(args.warmup):
+        run_step(rq, args.cap)
+    t1 = time.perf_counter()
+    rsteps = [run_step(rq, args.cap) for _ in range(args.steps)]
+    r_elapsed = time.perf_counter() - t1
+    line["reordered"] = {
+        "order": "candidate-aware (csm_query_reorder); value above uses the reference's BFS edge order",
+        "value": round(sum(r.count for r in rsteps[-1]) / (r_elapsed / args.steps), 1),
+        "ms_per_step": round(r_elapsed * 1e3 / args.steps, 3),
+        "counts_equal_default_order": [(r.count, r.capped) for r in rsteps[-1]] == [(r.count, r.capped) for r in res],
+    }
+    for q in rq:
         q.free()
     g.free()
     if not args.no_batched:

@@ -8,6 +8,7 @@
 //   --cap N           stop after N embeddings (count = min(total, N))
 //   --list            print the embeddings (query-id order, sorted) before num_res
 //   --no-refine       skip the cluster refinement
+//   --reorder         candidate-aware matching order (same counts)
 //   --arena-gb G      device arena size
 //   --batch FILE      FILE lists query paths; all run in one batched launch
 //   --stats           timing and kernel accounting as one JSON line on stderr
@@ -41,6 +42,7 @@ void print_help() {
       "  --cap UINT                  result cap (0 = none)\n"
       "  --list                      print the embeddings\n"
       "  --no-refine                 skip the cluster refinement\n"
+      "  --reorder                   candidate-aware matching order (same counts; capped lists differ)\n"
       "  --arena-gb FLOAT            device arena size in GiB (0 = 85%% of free HBM)\n"
       "  --batch TEXT                file with one query path per line (batched launch)\n"
       "  --stats                     JSON statistics on stderr\n");
@@ -79,7 +81,7 @@ int main(int argc, char **argv) {
   std::string qpath, dpath, batch_path;
   uint64_t gpu = 0, cap = 0;
   double arena_gb = 0;
-  bool have_q = false, have_d = false, list = false, refine = true, stats = false;
+  bool have_q = false, have_d = false, list = false, refine = true, stats = false, reorder = false;
   std::vector<std::string> extras;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -111,6 +113,8 @@ int main(int argc, char **argv) {
       batch_path = value("--batch");
     } else if (a == "--list") {
       list = true;
+    } else if (a == "--reorder") {
+      reorder = true;
     } else if (a == "--no-refine") {
       refine = false;
     } else if (a == "--stats") {
@@ -169,6 +173,10 @@ int main(int argc, char **argv) {
   for (size_t i = 0; i < plans.size(); ++i) {
     st = csm_query_prepare(ctx, g, &plans[i], &qs[i]);
     if (st != CSM_OK) die(st, qpaths[i].c_str(), csm_last_error(ctx));
+    if (reorder) {
+      st = csm_query_reorder(ctx, g, qs[i], refine ? 1 : 0);
+      if (st != CSM_OK) die(st, qpaths[i].c_str(), csm_last_error(ctx));
+    }
   }
 
   if (!batch_path.empty()) {

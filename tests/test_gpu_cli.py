@@ -69,3 +69,10 @@ def test_cli_binary_data_and_batch(tmp_path):
     assert r.returncode == 0, r.stderr
     got = [int(ln.split()[1]) for ln in r.stdout.splitlines() if ln.startswith("num_res: ")]
     assert got == single
+    # the candidate-aware order counts the same
+    r = run("-q", qpaths[0], "-d", str(binary), "--batch", str(listing), "--cap", cap, "--reorder")
+    assert r.returncode == 0, r.stderr
+    assert [int(ln.split()[1]) for ln in r.stdout.splitlines() if ln.startswith("num_res: ")] == single
+    for p, want in zip(qpaths, single):
+        r = run("-q", p, "-d", str(text), "--cap", cap, "--reorder")
+        assert r.returncode == 0 and last_count(r.stdout) == want, r.stderr
