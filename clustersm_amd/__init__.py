@@ -99,7 +99,7 @@ class KernelStat(C.Structure):
 class MatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("reuse_filter", C.c_uint32),
                 ("list", C.POINTER(C.c_uint32)), ("list_capacity", C.c_uint64),
-                ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reserved", C.c_uint32)]
+                ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reorder", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -118,7 +118,8 @@ class BatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("use_graph", C.c_uint32),
                 ("chunk_rows", C.c_uint32), ("wgs_per_cu", C.c_uint32), ("pool_bytes", C.c_uint64),
                 ("block_kb", C.c_uint32), ("max_out", C.c_uint32), ("ilp", C.c_uint32),
-                ("time_limit_ms", C.c_uint32), ("item_budget", C.c_uint64)]
+                ("time_limit_ms", C.c_uint32), ("item_budget", C.c_uint64), ("reorder", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ServeOpts(C.Structure):
@@ -451,12 +452,14 @@ class Query:
                              "csm_query_reorder")
 
     def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0,
-              time_limit_ms=0):
+              time_limit_ms=0, reorder=False):
         """Returns (Result, rows or None); rows are sorted, in query-id order.
-        With time_limit_ms, Result.status == E_TIMEOUT marks a stopped search."""
+        With time_limit_ms, Result.status == E_TIMEOUT marks a stopped search.
+        reorder: candidate-aware matching order after the filter (same counts)."""
         o = MatchOpts()
         o.cap, o.refine, o.reuse_filter, o.chunk_items = cap, 1 if refine else 0, 1 if reuse_filter else 0, chunk_items
         o.time_limit_ms = time_limit_ms
+        o.reorder = 1 if reorder else 0
         rows = None
         if want_list:
             lc = list_capacity if list_capacity is not None else (cap if cap else 1 << 20)
@@ -482,13 +485,17 @@ def batch_autotune(graph, queries, cap=0, refine=False, sample=32, time_limit_ms
 
 
 def batch_match(graph, queries, cap=0, refine=False, use_graph=True, chunk_rows=0, wgs_per_cu=0, pool_bytes=0,
-                block_kb=0, max_out=0, ilp=0, time_limit_ms=0, item_budget=0, opts=None):
+                block_kb=0, max_out=0, ilp=0, time_limit_ms=0, item_budget=0, opts=None, reorder=False):
     """All queries in one persistent kernel (csm_batch_match); counts only.
-    opts: a BatchOpts (e.g. from batch_autotune) used as given."""
+    opts: a BatchOpts (e.g. from batch_autotune) used as given, except that
+    reorder=True sets its reorder field."""
     arr = (_vp * len(queries))(*[q.h for q in queries])
     o = opts if opts is not None else BatchOpts(cap, 1 if refine else 0, 1 if use_graph else 0, chunk_rows,
                                                 wgs_per_cu, pool_bytes, block_kb, max_out, ilp, time_limit_ms,
                                                 item_budget)
+    if reorder:
+        o = BatchOpts.from_buffer_copy(o)
+        o.reorder = 1
     res = (Result * len(queries))()
     graph.ctx.check(lib().csm_batch_match(graph.ctx.h, graph.h, arr, len(queries), C.byref(o), res),
                     "csm_batch_match")

@@ -1222,6 +1222,11 @@ csm_status BatchRun::prepare(hipStream_t s) {
       queries_[i]->filtered = true;
     }
   }
+  if (o_.reorder)  // candidate-aware order from these counts: host work only
+    for (csm_query *q : queries_) {
+      const csm_status st = apply_reorder(ctx, g, q);
+      if (st != CSM_OK) return st;
+    }
   ms_filter_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
 
   // 2. candidate indexes of every query

@@ -62,6 +62,10 @@ csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q
 // of their lengths on ctx->stream; synchronises).
 csm_status slot_total(csm_ctx *ctx, const CandIndexHost &ci, uint64_t base, uint64_t n, uint64_t &total);
 
+// Candidate-aware matching order from q's filtered counts: rewrites q's plan
+// order and level descriptors (join.hip).
+csm_status apply_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q);
+
 // All queries' indexes with one launch per pass (one shared list array).
 csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
                                   std::vector<CandIndexHost> &out, std::vector<void *> &blocks, KernelTimer &kt);

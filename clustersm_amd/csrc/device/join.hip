@@ -852,6 +852,19 @@ csm_status plan_reorder_by_candidates(csm_query_plan &plan, const uint64_t *coun
   return CSM_OK;
 }
 
+csm_status apply_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q) {
+  csm_query_plan plan = q->plan;
+  csm_status st = plan_reorder_by_candidates(plan, q->counts);
+  if (st != CSM_OK) return set_error(ctx, st, "query is not connected");
+  QueryDesc d;
+  LevelDesc lv[CSM_MAX_QV];
+  st = query_build_desc(plan, g->n, d, lv);
+  if (st != CSM_OK) return set_error(ctx, st, "reordered plan is invalid");
+  q->plan = plan;
+  std::memcpy(q->level, lv, sizeof(lv));
+  return CSM_OK;
+}
+
 }  // namespace csm
 
 using namespace csm;
@@ -863,16 +876,7 @@ extern "C" csm_status csm_query_reorder(csm_ctx *ctx, const csm_graph *g, csm_qu
     const csm_status st = run_filter(ctx, g, q, refine);
     if (st != CSM_OK) return st;
   }
-  csm_query_plan plan = q->plan;
-  csm_status st = plan_reorder_by_candidates(plan, q->counts);
-  if (st != CSM_OK) return set_error(ctx, st, "query is not connected");
-  QueryDesc d;
-  LevelDesc lv[CSM_MAX_QV];
-  st = query_build_desc(plan, g->n, d, lv);
-  if (st != CSM_OK) return set_error(ctx, st, "reordered plan is invalid");
-  q->plan = plan;
-  std::memcpy(q->level, lv, sizeof(lv));
-  return CSM_OK;
+  return apply_reorder(ctx, g, q);
 }
 
 extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts *opts,
@@ -887,6 +891,13 @@ extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, 
   auto t0 = std::chrono::steady_clock::now();
   if (!(o.reuse_filter && q->filtered)) {
     csm_status st = run_filter(ctx, g, q, o.refine);
+    if (st != CSM_OK) {
+      out->status = st;
+      return st;
+    }
+  }
+  if (o.reorder) {
+    const csm_status st = apply_reorder(ctx, g, q);
     if (st != CSM_OK) {
       out->status = st;
       return st;

@@ -185,7 +185,9 @@ typedef struct csm_match_opts {
   uint64_t chunk_items;    /* 0 = autotuned */
   uint32_t time_limit_ms;  /* 0 = none; else the search stops after this and the result's status is
                               CSM_E_TIMEOUT with the count so far (the call itself returns CSM_OK) */
-  uint32_t reserved;
+  uint32_t reorder;        /* 1: candidate-aware matching order after the filter, as csm_query_reorder
+                              (same counts; a capped list holds the first embeddings of that order;
+                              the query keeps the new order) */
 } csm_match_opts;
 
 /* Kernel classes for per-kernel accounting. */
@@ -273,6 +275,9 @@ typedef struct csm_batch_opts {
                               status CSM_E_TIMEOUT and its count so far, as with csm_match */
   uint64_t item_budget;    /* items per query before the plain path takes it over; 0 = 5e9 without a
                               time limit, unlimited with one */
+  uint32_t reorder;        /* 1: candidate-aware matching order for every query, from the batch's own
+                              filter counts (as csm_query_reorder; same counts; the queries keep it) */
+  uint32_t reserved;
 } csm_batch_opts;
 csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
                            const csm_batch_opts *opts, csm_result *out);

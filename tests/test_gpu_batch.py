@@ -170,3 +170,24 @@ def test_autotune_picks_a_valid_setting(ctx):
     for q in qs:
         q.free()
     g.free()
+
+
+def test_reorder_option_keeps_counts(ctx):
+    """The reorder option (candidate-aware order from each path's own filter
+    counts) gives the default order's counts and cap flags, batched and plain."""
+    hg = csm.gen_rmat(12, 40000, 8, 1.0, 9)
+    g = ctx.upload(hg)
+    host = g.to_host()
+    r = random.Random(12)
+    plans = [csm.gen_query(host, r.randint(3, 14), csm.QUERY_AVG_DEGREE, 1200 + i, avg_degree=r.uniform(2, 3.5)).plan()
+             for i in range(16)]
+    cap = 500_000
+    qs = [g.prepare(p) for p in plans]
+    want = [(x.count, x.capped) for x in (q.match(cap=cap)[0] for q in qs)]
+    qs2 = [g.prepare(p) for p in plans]
+    res = csm.batch_match(g, qs2, cap=cap, reorder=True, time_limit_ms=20000)
+    assert [(x.count, x.capped) for x in res] == want
+    assert [(x.count, x.capped) for x in (q.match(cap=cap, reorder=True)[0] for q in qs2)] == want
+    for q in qs + qs2:
+        q.free()
+    g.free()
