@@ -456,7 +456,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
+        print(json.dumps({"error": "the library runs on one GPU; multi-GPU sharding is a design only "
                                    "(queries or root candidates split across GPUs, one u64 count reduction)"}))
         sys.exit(2)
 
