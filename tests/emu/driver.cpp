@@ -1,6 +1,9 @@
 // Runs the library, built for the CPU emulation, on graph files:
 //   driver <data.graph> <cap> <query.graph>...
-// prints "plain <count> <capped>" and "batch <count> <capped>" per query.
+// prints per query "plain <count> <capped>", then its list under a cap of 64
+// ("list <rows>" and one "row ..." line per embedding, sorted, query-id
+// order), "batch <count> <capped>", and with the candidate-aware order
+// (reorder option) "plainr ..." and "batchr ...".
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -24,23 +27,38 @@ int main(int argc, char **argv) {
   csm_graph *g = nullptr;
   check(csm_graph_upload(ctx, hd, &g), "upload", ctx);
   std::vector<csm_query *> qs;
-  static csm_query_plan plan;
+  std::vector<csm_query_plan> plans(static_cast<size_t>(argc - 3));
   for (int i = 3; i < argc; ++i) {
     csm_host_graph *hq = nullptr;
     check(csm_host_graph_read(argv[i], &hq), "read query", nullptr);
-    check(csm_plan_build(hq, &plan), "plan", nullptr);
+    check(csm_plan_build(hq, &plans[i - 3]), "plan", nullptr);
     csm_query *q = nullptr;
-    check(csm_query_prepare(ctx, g, &plan, &q), "prepare", ctx);
+    check(csm_query_prepare(ctx, g, &plans[i - 3], &q), "prepare", ctx);
     qs.push_back(q);
     csm_host_graph_free(hq);
   }
-  for (csm_query *q : qs) {
+  for (size_t i = 0; i < qs.size(); ++i) {
+    csm_query *q = qs[i];
     csm_match_opts o{};
     o.cap = cap;
     o.refine = 1;
     csm_result r{};
     check(csm_match(ctx, g, q, &o, &r), "match", ctx);
     std::printf("plain %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+    const uint32_t nq = plans[i].num_v;
+    std::vector<uint32_t> rows(64ull * nq);
+    csm_match_opts lo{};
+    lo.cap = 64;
+    lo.refine = 1;
+    lo.list = rows.data();
+    lo.list_capacity = 64;
+    check(csm_match(ctx, g, q, &lo, &r), "list", ctx);
+    std::printf("list %llu\n", static_cast<unsigned long long>(r.listed));
+    for (uint64_t k = 0; k < r.listed; ++k) {
+      std::printf("row");
+      for (uint32_t j = 0; j < nq; ++j) std::printf(" %u", rows[k * nq + j]);
+      std::printf("\n");
+    }
   }
   csm_batch_opts bo{};
   bo.cap = cap;
@@ -48,6 +66,26 @@ int main(int argc, char **argv) {
   std::vector<csm_result> res(qs.size());
   check(csm_batch_match(ctx, g, qs.data(), static_cast<uint32_t>(qs.size()), &bo, res.data()), "batch", ctx);
   for (const csm_result &r : res) std::printf("batch %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+  // the candidate-aware order, on fresh copies of the queries
+  std::vector<csm_query *> qr;
+  for (const csm_query_plan &p : plans) {
+    csm_query *q = nullptr;
+    check(csm_query_prepare(ctx, g, &p, &q), "prepare", ctx);
+    qr.push_back(q);
+  }
+  bo.reorder = 1;
+  check(csm_batch_match(ctx, g, qr.data(), static_cast<uint32_t>(qr.size()), &bo, res.data()), "batch reorder", ctx);
+  for (const csm_result &r : res) std::printf("batchr %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+  for (csm_query *q : qr) {
+    csm_match_opts o{};
+    o.cap = cap;
+    o.refine = 1;
+    o.reorder = 1;
+    csm_result r{};
+    check(csm_match(ctx, g, q, &o, &r), "match reorder", ctx);
+    std::printf("plainr %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+    csm_query_free(ctx, q);
+  }
   for (csm_query *q : qs) csm_query_free(ctx, q);
   csm_graph_free(ctx, g);
   csm_host_graph_free(hd);

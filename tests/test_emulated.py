@@ -6,8 +6,10 @@ and wave operations as meeting points (tests/emu/emu_rt.cpp).  Every global
 allocation is its own heap block, so an out-of-bounds access in a kernel is
 reported with its stack.  The plain path and the persistent batched kernel
 run on small random graphs and queries (with non-tree edges) and must give
-the oracle's counts, capped and uncapped.  Test infrastructure only; the
-product is the gfx950 build."""
+the oracle's counts, capped and uncapped, also with the candidate-aware
+order (reorder option), and the plain path's list under a cap of 64 (sorted
+on the device) must be the oracle's.  Test infrastructure only; the product
+is the gfx950 build."""
 import os
 import random
 import subprocess
@@ -42,6 +44,33 @@ def _oracle(dl, de, queries, cap):
     return out
 
 
+def _oracle_lists(dl, de, queries, cap=64):
+    d = O.Data.from_arrays(dl, de)
+    out = []
+    for ql, qe in queries:
+        oq = O.Query.from_arrays(ql, qe)
+        _, _, rows, _, _ = O.match(d, oq, oq.plan(), cap=cap, want_list=True, list_cap=cap, threads=1)
+        out.append(sorted(tuple(int(x) for x in r) for r in rows.tolist()))  # the oracle lists in matching order
+    return out
+
+
+def _parse(stdout):
+    """Per-kind results and the listed rows of each query, from the driver."""
+    got = {"plain": [], "batch": [], "plainr": [], "batchr": []}
+    lists = []
+    for ln in stdout.split("\n"):
+        f = ln.split()
+        if not f:
+            continue
+        if f[0] in got:
+            got[f[0]].append((int(f[1]), f[2] == "1"))
+        elif f[0] == "list":
+            lists.append([])
+        elif f[0] == "row":
+            lists[-1].append(tuple(int(x) for x in f[1:]))
+    return got, lists
+
+
 @pytest.mark.parametrize("seed", [3, 4])
 def test_device_code_on_cpu_matches_oracle(driver, tmp_path, seed):
     r = random.Random(seed)
@@ -62,12 +91,11 @@ def test_device_code_on_cpu_matches_oracle(driver, tmp_path, seed):
                              timeout=600)
         assert out.returncode == 0, out.stderr[-3000:]
         assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
-        lines = out.stdout.split("\n")
-        plain = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("plain")]
-        batch = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("batch")]
+        got, lists = _parse(out.stdout)
         want = _oracle(dl, de, queries, cap)
-        assert plain == want, (cap, plain, want)
-        assert batch == want, (cap, batch, want)
+        for kind in ("plain", "batch", "plainr", "batchr"):
+            assert got[kind] == want, (cap, kind, got[kind], want)
+        assert lists == _oracle_lists(dl, de, queries), cap
 
 
 def test_heavy_candidate_lists_on_cpu(driver, tmp_path):
@@ -110,9 +138,8 @@ def test_heavy_candidate_lists_on_cpu(driver, tmp_path):
                              timeout=900)
         assert out.returncode == 0, out.stderr[-3000:]
         assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
-        lines = out.stdout.split("\n")
-        plain = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("plain")]
-        batch = [(int(l.split()[1]), l.split()[2] == "1") for l in lines if l.startswith("batch")]
+        got, lists = _parse(out.stdout)
         want = _oracle(dl, de, queries, cap)
-        assert plain == want, (cap, plain, want)
-        assert batch == want, (cap, batch, want)
+        for kind in ("plain", "batch", "plainr", "batchr"):
+            assert got[kind] == want, (cap, kind, got[kind], want)
+        assert lists == _oracle_lists(dl, de, queries), cap
