@@ -9,11 +9,7 @@ import pytest
 
 import clustersm_amd as csm
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the first run of this module after the batch module ended in SIGABRT "
-                            "(cause not found yet); serving stays off the GPU until it is"),
-]
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
