@@ -256,6 +256,7 @@ __global__ void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, 
 // anchor's index; dscan[n] = 0.
 // At the count level, when the last position's parent is an earlier column
 // (fused_rows): fbase/fbeg/flen[i] = that parent's list of the last position
+// (fbeg/flen only when the count kernel reads them: store_list)
 // and its length minus the row's own members.  With row_count (one anchor,
 // and no per-item correction), the row's count is its driver list minus the
 // row's own ids, times fbase when fused: it goes to partials[block] and the
@@ -263,6 +264,7 @@ __global__ void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, 
 struct RowPass {
   LevelJoin lv_last;
   uint32_t fused_rows, row_count;
+  uint32_t store_list;  // fbeg/flen are read by the count kernel (the last label equals position k's)
   uint32_t *fbase;
   uint64_t *fbeg;
   uint32_t *flen;
@@ -304,8 +306,10 @@ __global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uin
       }
       fb = static_cast<uint32_t>(c);
       rp.fbase[i] = fb;
-      rp.fbeg[i] = lb;
-      rp.flen[i] = static_cast<uint32_t>(le - lb);
+      if (rp.store_list) {
+        rp.fbeg[i] = lb;
+        rp.flen[i] = static_cast<uint32_t>(le - lb);
+      }
     }
     if (rp.row_count) {
       uint64_t surv = best;
@@ -581,6 +585,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
       rp.fbase = ws.fbase;
       rp.fbeg = ws.fbeg;
       rp.flen = ws.flen;
+      rp.store_list = last_inj_k;
       rp.partials = ws.row_sums;
     }
     const uint32_t grid = grid_for(nrows[k] + 1);
@@ -604,7 +609,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     anchor_len_kernel<<<grid, 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k], ws.drv[k], rp);
     scan_u64_exclusive(ws.dscan[k], nrows[k] + 1, ws.partials, nullptr, s);
     choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], 0, item_cap, ws.ctl + kCtlChunkA);
-    kt.end(CSM_K_AUX, nrows[k] * (per_row + 9 + (count_level ? 16 : 0)));
+    kt.end(CSM_K_AUX, nrows[k] * (per_row + 9 + (count_level ? (last_inj_k ? 16 : 4) : 0)));
     CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     cursor[k] = 0;
