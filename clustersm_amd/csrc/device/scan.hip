@@ -65,36 +65,41 @@ __global__ __launch_bounds__(kScanThreads) void partials_scan(uint64_t *partials
 }
 
 // Each thread owns kPerThread consecutive elements (blocked layout), so the
-// scan order is the element order.
+// scan order is the element order.  The staging array holds one pad element
+// after every kPerThread: a thread's block then starts kPerThread + 1
+// elements after its neighbour's, so the blocked accesses of a wave spread
+// over the LDS banks instead of all meeting in the same ones.
+__device__ inline uint32_t padded(uint32_t e) { return e + e / kPerThread; }
+
 __global__ __launch_bounds__(kScanThreads) void tile_scan(uint64_t *data, uint64_t n, const uint64_t *partials) {
   __shared__ uint64_t lds[kScanThreads / kWave];
-  __shared__ uint64_t stage[kScanTile];
+  __shared__ uint64_t stage[kScanTile + kScanTile / kPerThread];
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
     uint64_t idx = base + static_cast<uint64_t>(i) * kScanThreads + threadIdx.x;
-    stage[i * kScanThreads + threadIdx.x] = idx < n ? data[idx] : 0;
+    stage[padded(i * kScanThreads + threadIdx.x)] = idx < n ? data[idx] : 0;
   }
   __syncthreads();
   uint64_t v[kPerThread];
   uint64_t s = 0;
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
-    v[i] = stage[threadIdx.x * kPerThread + i];
+    v[i] = stage[padded(threadIdx.x * kPerThread + i)];
     s += v[i];
   }
   uint64_t total;
   uint64_t run = block_exclusive_scan(s, lds, &total) + partials[blockIdx.x];
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
-    stage[threadIdx.x * kPerThread + i] = run;
+    stage[padded(threadIdx.x * kPerThread + i)] = run;
     run += v[i];
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
     uint64_t idx = base + static_cast<uint64_t>(i) * kScanThreads + threadIdx.x;
-    if (idx < n) data[idx] = stage[i * kScanThreads + threadIdx.x];
+    if (idx < n) data[idx] = stage[padded(i * kScanThreads + threadIdx.x)];
   }
 }
 
