@@ -35,6 +35,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 /* ------------------------------------------------------------------------ */
 /* tokenizer for the t/v/e text format                                       */
@@ -202,51 +203,114 @@ int oc_query_read(const char *path, oc_query *q) {
 /* data graph: simple, sorted adjacency                                      */
 /* ------------------------------------------------------------------------ */
 
+/* CSR build, one thread per vertex range: each thread streams the whole
+ * edge list and places the entries whose owner falls in its range, then
+ * sorts and deduplicates its lists; a final pass compacts them. */
+typedef struct {
+  const uint32_t *edges;
+  uint64_t m;
+  uint32_t lo, hi;           /* vertex range [lo, hi) */
+  const uint64_t *off;       /* raw list starts (duplicates included) */
+  uint32_t *adj;
+  uint64_t *len;             /* per vertex: entries after dedup */
+  uint64_t *deg;             /* per vertex: raw degree (pass 0) */
+  int pass;
+} oc_csr_job;
+
 static int cmp_u32(const void *a, const void *b) {
   uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
   return (x > y) - (x < y);
 }
 
+static void *oc_csr_worker(void *arg) {
+  oc_csr_job *j = (oc_csr_job *)arg;
+  const uint32_t lo = j->lo, hi = j->hi;
+  if (j->pass == 0) {  /* degrees of the range */
+    for (uint64_t k = 0; k < j->m; k++) {
+      const uint32_t s = j->edges[2 * k], d = j->edges[2 * k + 1];
+      if (s == d) continue;
+      if (s >= lo && s < hi) j->deg[s]++;
+      if (d >= lo && d < hi) j->deg[d]++;
+    }
+    return NULL;
+  }
+  for (uint32_t v = lo; v < hi; v++) j->len[v] = 0;
+  for (uint64_t k = 0; k < j->m; k++) {
+    const uint32_t s = j->edges[2 * k], d = j->edges[2 * k + 1];
+    if (s == d) continue;
+    if (s >= lo && s < hi) j->adj[j->off[s] + j->len[s]++] = d;
+    if (d >= lo && d < hi) j->adj[j->off[d] + j->len[d]++] = s;
+  }
+  for (uint32_t v = lo; v < hi; v++) {
+    uint32_t *a = j->adj + j->off[v];
+    const uint64_t e = j->len[v];
+    qsort(a, (size_t)e, 4, cmp_u32);
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < e; i++)
+      if (i == 0 || a[i] != a[i - 1]) a[w++] = a[i];
+    j->len[v] = w;
+  }
+  return NULL;
+}
+
 int oc_data_from_arrays(uint32_t n, const uint32_t *labels, uint64_t m,
                         const uint32_t *edges, oc_data **out) {
+  for (uint64_t k = 0; k < 2 * m; k++)
+    if (edges[k] >= n) return OC_E_RANGE;
   oc_data *g = (oc_data *)calloc(1, sizeof(oc_data));
   if (!g) return OC_E_NOMEM;
   g->n = n;
   g->label = (uint32_t *)malloc((size_t)(n ? n : 1) * 4);
   g->off = (uint64_t *)calloc((size_t)n + 1, 8);
   uint64_t *deg = (uint64_t *)calloc((size_t)n + 1, 8);
-  if (!g->label || !g->off || !deg) { free(deg); oc_data_free(g); return OC_E_NOMEM; }
+  uint64_t *len = (uint64_t *)calloc((size_t)n + 1, 8);
+  if (!g->label || !g->off || !deg || !len) { free(deg); free(len); oc_data_free(g); return OC_E_NOMEM; }
   memcpy(g->label, labels, (size_t)n * 4);
-  for (uint64_t k = 0; k < m; k++) {
-    uint32_t s = edges[2 * k], d = edges[2 * k + 1];
-    if (s >= n || d >= n) { free(deg); oc_data_free(g); return OC_E_RANGE; }
-    if (s == d) continue;
-    deg[s]++;
-    deg[d]++;
+  int nt = 1;
+  if (m >= (1u << 20)) {
+    long c = sysconf(_SC_NPROCESSORS_ONLN);
+    nt = c < 1 ? 1 : c > 16 ? 16 : (int)c;
   }
-  for (uint32_t v = 0; v < n; v++) g->off[v + 1] = g->off[v] + deg[v];
-  g->adj = (uint32_t *)malloc((size_t)(g->off[n] ? g->off[n] : 1) * 4);
-  if (!g->adj) { free(deg); oc_data_free(g); return OC_E_NOMEM; }
-  memset(deg, 0, (size_t)(n + 1) * 8);
-  for (uint64_t k = 0; k < m; k++) {
-    uint32_t s = edges[2 * k], d = edges[2 * k + 1];
-    if (s == d) continue;
-    g->adj[g->off[s] + deg[s]++] = d;
-    g->adj[g->off[d] + deg[d]++] = s;
+  oc_csr_job jobs[16];
+  pthread_t th[16];
+  for (int t = 0; t < nt; t++) {
+    jobs[t].edges = edges;
+    jobs[t].m = m;
+    jobs[t].lo = (uint32_t)((uint64_t)n * (uint64_t)t / (uint64_t)nt);
+    jobs[t].hi = (uint32_t)((uint64_t)n * (uint64_t)(t + 1) / (uint64_t)nt);
+    jobs[t].off = g->off;
+    jobs[t].len = len;
+    jobs[t].deg = deg;
   }
-  /* sort + dedup each list, then compact */
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      for (uint32_t v = 0; v < n; v++) g->off[v + 1] = g->off[v] + deg[v];
+      g->adj = (uint32_t *)malloc((size_t)(g->off[n] ? g->off[n] : 1) * 4);
+      if (!g->adj) { free(deg); free(len); oc_data_free(g); return OC_E_NOMEM; }
+    }
+    for (int t = 0; t < nt; t++) {
+      jobs[t].pass = pass;
+      jobs[t].adj = g->adj;
+    }
+    if (nt == 1) {
+      oc_csr_worker(&jobs[0]);
+    } else {
+      for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, oc_csr_worker, &jobs[t]);
+      for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+    }
+  }
+  /* compact the deduplicated lists */
   uint64_t w = 0;
   for (uint32_t v = 0; v < n; v++) {
-    uint64_t b = g->off[v], e = g->off[v + 1];
-    qsort(g->adj + b, (size_t)(e - b), 4, cmp_u32);
-    uint64_t start = w;
-    for (uint64_t i = b; i < e; i++)
-      if (i == b || g->adj[i] != g->adj[i - 1]) g->adj[w++] = g->adj[i];
-    g->off[v] = start;
+    const uint64_t b = g->off[v];
+    if (w != b) memmove(g->adj + w, g->adj + b, (size_t)len[v] * 4);
+    g->off[v] = w;
+    w += len[v];
   }
   g->off[n] = w;
   g->m = w / 2;
   free(deg);
+  free(len);
   *out = g;
   return OC_OK;
 }
