@@ -167,6 +167,7 @@ SIGNATURES = {
     "csm_candidates_download": ([_vp, _vp, _u64p], C.c_int),
     "csm_query_reorder": ([_vp, _vp, _vp, C.c_int], C.c_int),
     "csm_match": ([_vp, _vp, _vp, C.POINTER(MatchOpts), C.POINTER(Result)], C.c_int),
+    "csm_match_noalloc": ([_vp, _vp, _vp, C.POINTER(MatchOpts), C.POINTER(Result)], C.c_int),
     "csm_batch_match": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.POINTER(Result)], C.c_int),
     "csm_batch_autotune": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.c_uint32,
                             C.POINTER(BatchOpts)], C.c_int),
@@ -472,6 +473,19 @@ class Query:
         if want_list:
             rows = rows[: r.listed]
         return r, rows
+
+
+    def match_noalloc(self, cap=0, refine=True, reuse_filter=False, time_limit_ms=0, reorder=False):
+        """csm_match_noalloc: count only, every workspace buffer in the library's
+        static device scratch (no allocation).  Returns the Result."""
+        o = MatchOpts()
+        o.cap, o.refine, o.reuse_filter = cap, 1 if refine else 0, 1 if reuse_filter else 0
+        o.time_limit_ms = time_limit_ms
+        o.reorder = 1 if reorder else 0
+        r = Result()
+        self.graph.ctx.check(lib().csm_match_noalloc(self.graph.ctx.h, self.graph.h, self.h, C.byref(o),
+                                                     C.byref(r)), "csm_match_noalloc")
+        return r
 
 
 def batch_autotune(graph, queries, cap=0, refine=False, sample=32, time_limit_ms=0):

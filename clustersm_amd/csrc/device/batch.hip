@@ -39,7 +39,6 @@
 
 namespace csm {
 
-csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts &o, csm_result &res);
 
 namespace {
 
@@ -1154,7 +1153,7 @@ struct StreamSwap {
 };
 
 BatchRun::BatchRun(csm_ctx *ctx, const csm_graph *g, std::vector<csm_query *> queries, const csm_batch_opts &o)
-    : ctx_(ctx), g_(g), queries_(std::move(queries)), o_(o), kstat_(CSM_K_NUM) {
+    : ctx_(ctx), g_(g), queries_(std::move(queries)), o_(o), mem_(ctx), kstat_(CSM_K_NUM) {
   t0_ = std::chrono::steady_clock::now();
   words_ = queries_.empty() ? 0 : queries_[0]->desc.words;
   for (auto *q : queries_) max_nq_ = std::max(max_nq_, q->desc.nq);
@@ -1172,14 +1171,10 @@ BatchRun::~BatchRun() {
   release_graph();
   join_kt_.reset();
   hipStreamSynchronize(ctx_->stream);
-  for (void *p : blocks_) ctx_->arena.free(p);
+  mem_.release();
 }
 
-void *BatchRun::get(uint64_t bytes) {
-  void *p = ctx_->arena.alloc(bytes);
-  if (p) blocks_.push_back(p);
-  return p;
-}
+void *BatchRun::get(uint64_t bytes) { return mem_.get(bytes); }
 
 csm_status BatchRun::prepare(hipStream_t s) {
   StreamSwap swap(ctx_, s);
@@ -1232,7 +1227,7 @@ csm_status BatchRun::prepare(hipStream_t s) {
   // 2. candidate indexes of every query
   {
     std::vector<const csm_query *> cq(queries_.begin(), queries_.end());
-    csm_status st = build_cand_index_batch(ctx, g, cq, cih_, blocks_, kt);
+    csm_status st = build_cand_index_batch(ctx, g, cq, cih_, mem_, kt);
     if (st != CSM_OK) return st;
   }
 

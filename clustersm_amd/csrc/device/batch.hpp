@@ -35,7 +35,7 @@ class BatchRun {
   const csm_graph *g_;
   std::vector<csm_query *> queries_;
   csm_batch_opts o_;
-  std::vector<void *> blocks_;
+  WorkMem mem_;  // arena blocks of this batch
   hipStream_t join_stream_ = nullptr;
   std::chrono::steady_clock::time_point t0_;
   double ms_filter_ = 0;

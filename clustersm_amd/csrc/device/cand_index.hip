@@ -234,17 +234,13 @@ uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(
 }  // namespace
 
 csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
-                                  std::vector<CandIndexHost> &out, std::vector<void *> &blocks, KernelTimer &kt) {
+                                  std::vector<CandIndexHost> &out, WorkMem &mem, KernelTimer &kt) {
   hipStream_t s = ctx->stream;
   const uint32_t nqs = static_cast<uint32_t>(qs.size());
   if (nqs == 0) return CSM_OK;
   const uint32_t words = qs[0]->desc.words;
   out.assign(nqs, CandIndexHost());
-  auto get = [&](uint64_t bytes) {
-    void *p = ctx->arena.alloc(bytes);
-    if (p) blocks.push_back(p);
-    return p;
-  };
+  auto get = [&](uint64_t bytes) { return mem.get(bytes); };
   // rank prefixes of every query in one buffer
   std::vector<uint64_t> rank_base(nqs);
   uint64_t rank_n = 0;
@@ -295,7 +291,7 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   // the free arena; when the pairs do not fit, every list goes through the
   // per-word walk (the bounds are the same either way)
   uint64_t heavy_cap = std::min<uint64_t>(static_cast<uint64_t>(g->num_heavy) * slots.size(),
-                                          std::max<uint64_t>(1ull << 16, ctx->arena.largest_free() / 64 / sizeof(HeavyPair)));
+                                          std::max<uint64_t>(mem.fixed() ? 0 : 1ull << 16, mem.largest_free() / 64 / sizeof(HeavyPair)));
   HeavyPair *heavy = heavy_cap ? static_cast<HeavyPair *>(get(sizeof(HeavyPair) * heavy_cap)) : nullptr;
   if (!heavy) heavy_cap = 0;
   std::vector<RankJob> jobs(nqs);
@@ -365,9 +361,9 @@ csm_status slot_total(csm_ctx *ctx, const CandIndexHost &ci, uint64_t base, uint
 }
 
 csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
-                            std::vector<void *> &blocks, KernelTimer &kt) {
+                            WorkMem &mem, KernelTimer &kt) {
   std::vector<CandIndexHost> out;
-  csm_status st = build_cand_index_batch(ctx, g, {q}, out, blocks, kt);
+  csm_status st = build_cand_index_batch(ctx, g, {q}, out, mem, kt);
   if (st == CSM_OK) ci = out[0];
   return st;
 }

@@ -53,14 +53,19 @@ struct CandIndexHost {
   LevelJoin level[CSM_MAX_QV];
 };
 
-// Builds the index of q (candidates must be filtered) with arena memory owned
-// by `blocks` (freed by the caller).  Launches on ctx->stream and synchronises.
+// Builds the index of q (candidates must be filtered) in memory from `mem`
+// (held by the caller).  Launches on ctx->stream and synchronises.
 csm_status build_cand_index(csm_ctx *ctx, const csm_graph *g, const csm_query *q, CandIndexHost &ci,
-                            std::vector<void *> &blocks, KernelTimer &kt);
+                            WorkMem &mem, KernelTimer &kt);
 
 // Total length of the n lists of a slot starting at entry `base` (host copy
 // of their lengths on ctx->stream; synchronises).
 csm_status slot_total(csm_ctx *ctx, const CandIndexHost &ci, uint64_t base, uint64_t n, uint64_t &total);
+
+// The plain path's join of one filtered query (join.hip); its workspace comes
+// from `mem` (nullptr: the context arena).
+csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts &o, csm_result &res,
+                    WorkMem *mem = nullptr);
 
 // Candidate-aware matching order from q's filtered counts: rewrites q's plan
 // order and level descriptors (join.hip).
@@ -68,6 +73,6 @@ csm_status apply_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q);
 
 // All queries' indexes with one launch per pass (one shared list array).
 csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
-                                  std::vector<CandIndexHost> &out, std::vector<void *> &blocks, KernelTimer &kt);
+                                  std::vector<CandIndexHost> &out, WorkMem &mem, KernelTimer &kt);
 
 }  // namespace csm

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -163,6 +164,50 @@ inline csm_status set_error(csm_ctx *ctx, csm_status st, const std::string &msg)
   if (ctx) ctx->last_error = msg;
   return st;
 }
+
+// Device memory for one call's workspace: either sub-allocations of the
+// context arena (returned to it when the WorkMem goes away), or a bump region
+// over a fixed device buffer (nothing allocated, nothing returned).
+class WorkMem {
+ public:
+  explicit WorkMem(csm_ctx *ctx) : ctx_(ctx) {}
+  WorkMem(csm_ctx *ctx, void *region, uint64_t size)
+      : ctx_(ctx), region_(static_cast<char *>(region)), size_(size) {}
+  ~WorkMem() { release(); }
+  WorkMem(const WorkMem &) = delete;
+  WorkMem &operator=(const WorkMem &) = delete;
+  void *get(uint64_t bytes) {
+    bytes = (std::max<uint64_t>(bytes, 1) + 255) / 256 * 256;
+    if (region_) {
+      if (bytes > size_ - used_) return nullptr;
+      void *p = region_ + used_;
+      used_ += bytes;
+      return p;
+    }
+    void *p = ctx_->arena.alloc(bytes);
+    if (p) blocks_.push_back(p);
+    return p;
+  }
+  uint64_t largest_free() const { return region_ ? size_ - used_ : ctx_->arena.largest_free(); }
+  bool fixed() const { return region_ != nullptr; }
+  uint64_t capacity() const { return size_; }
+  void release() {
+    for (void *p : blocks_) ctx_->arena.free(p);
+    blocks_.clear();
+    used_ = 0;
+  }
+
+ private:
+  csm_ctx *ctx_;
+  char *region_ = nullptr;
+  uint64_t size_ = 0, used_ = 0;
+  std::vector<void *> blocks_;
+};
+
+// The library's static device scratch (runtime.hip): base address on
+// `device`, resolved once per device at csm_open; nullptr before that.
+constexpr uint64_t kScratchBytes = CSM_NOALLOC_SCRATCH_BYTES;
+void *scratch_region(int device);
 
 #define CSM_HIP_TRY(ctx, expr)                                                             \
   do {                                                                                     \

@@ -346,17 +346,15 @@ FilterQ filter_entry(const csm_graph *g, csm_query *q, uint64_t *stats) {
 }
 
 // Filter + refinement for one query on ctx->stream; fills q->counts.
-csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine) {
+csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine, WorkMem *mem) {
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
   const uint32_t words = q->desc.words, nq = q->desc.nq;
-  uint64_t *stats = static_cast<uint64_t *>(ctx->arena.alloc(8 * kStatSize));
-  FilterQ *d_q = static_cast<FilterQ *>(ctx->arena.alloc(sizeof(FilterQ)));
-  if (!stats || !d_q) {
-    ctx->arena.free(stats);
-    ctx->arena.free(d_q);
-    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (filter)");
-  }
+  WorkMem own(ctx);
+  WorkMem &wm = mem ? *mem : own;
+  uint64_t *stats = static_cast<uint64_t *>(wm.get(8 * kStatSize));
+  FilterQ *d_q = static_cast<FilterQ *>(wm.get(sizeof(FilterQ)));
+  if (!stats || !d_q) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (filter)");
   FilterQ fq = filter_entry(g, q, stats);
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_q, &fq, sizeof(FilterQ), hipMemcpyHostToDevice, s));
   std::memset(q->kstat, 0, sizeof(q->kstat));
@@ -379,8 +377,6 @@ csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine
     q->kstat[CSM_K_REFINE].bytes = refine ? 12ull * host[kStatRefineScanned] + clusters_bytes : 0;
     q->kstat[CSM_K_AUX].bytes += 8ull * words * nq;
   }
-  ctx->arena.free(stats);
-  ctx->arena.free(d_q);
   q->filtered = true;
   q->ms_filter = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return CSM_OK;

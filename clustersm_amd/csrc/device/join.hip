@@ -32,10 +32,9 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "cand_index.hpp"
+#include "filter.hpp"
 
 namespace csm {
-
-csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine);
 
 namespace {
 
@@ -425,8 +424,7 @@ void sort_rows_host(std::vector<uint32_t> &qrows, uint64_t nl, uint32_t nq, uint
 }
 
 struct Workspace {
-  csm_ctx *ctx = nullptr;
-  std::vector<void *> blocks;
+  WorkMem *mem = nullptr;
   uint32_t *rows[CSM_MAX_QV + 1] = {};
   uint64_t *dscan[CSM_MAX_QV + 1] = {};
   uint8_t *drv[CSM_MAX_QV + 1] = {};
@@ -434,19 +432,13 @@ struct Workspace {
   uint32_t *tile_row = nullptr;
   uint32_t *fbase = nullptr, *flen = nullptr;  // count level, per row
   uint64_t *fbeg = nullptr, *row_sums = nullptr;
-  ~Workspace() {
-    for (void *p : blocks) ctx->arena.free(p);
-  }
-  void *get(uint64_t bytes) {
-    void *p = ctx->arena.alloc(bytes);
-    if (p) blocks.push_back(p);
-    return p;
-  }
+  void *get(uint64_t bytes) { return mem->get(bytes); }
 };
 
 }  // namespace
 
-csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts &o, csm_result &res) {
+csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts &o, csm_result &res,
+                    WorkMem *mem) {
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
   const uint32_t nq = q->desc.nq;
@@ -477,8 +469,10 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     finish_time();
     return CSM_OK;
   }
+  WorkMem own(ctx);
+  WorkMem &wm = mem ? *mem : own;
   Workspace ws;
-  ws.ctx = ctx;
+  ws.mem = &wm;
   // chunk size: requested, else what fits in half of the largest free block
   uint64_t per_item = 16;  // mask bit, tile counter, scan partials, rounded up
   for (uint32_t k = 2; k <= stop; ++k) per_item += 4ull * k + 8 + 1;
@@ -486,7 +480,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   if (fused_rows) per_item += 16;
   // candidate adjacency index first: its size is data dependent
   CandIndexHost cih;
-  csm_status st = build_cand_index(ctx, g, q, cih, ws.blocks, kt);
+  csm_status st = build_cand_index(ctx, g, q, cih, wm, kt);
   if (st != CSM_OK) return st;
   CandIndex ci{q->bits, cih.rank_pref, cih.cn_off, cih.cn_len, cih.cn_adj, words};
   const LevelJoin *lj = cih.level;
@@ -496,7 +490,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   for (uint32_t t = 0; t < lj[L].ninj; ++t) last_inj_k |= lj[L].inj[t] == stop;
   const bool row_count = fused_rows && !last_inj_k && lj[stop].nanchor == 1;
   const uint64_t fixed = 13ull * (n1 + 1) + 8ull * (words + 1) + (fused_rows ? 16ull * (n1 + 1) : 0) + (2 << 20);
-  const uint64_t avail = ctx->arena.largest_free();
+  const uint64_t avail = wm.largest_free();
   if (avail < fixed + per_item * g->max_degree * 2)
     return set_error(ctx, CSM_E_NOMEM, "arena too small for the join workspace");
   uint64_t item_cap = o.chunk_items ? o.chunk_items : (64ull << 20);
@@ -911,6 +905,43 @@ extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, 
   for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
   out->ms_filter = q->ms_filter;
   csm_status st = run_join(ctx, g, q, o, *out);
+  if (st != CSM_OK || out->status != CSM_E_TIMEOUT) out->status = st;
+  out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+// csm_match (count only) with every workspace buffer in the library's static
+// device scratch: no hipMalloc, no arena block, no allocating library call.
+extern "C" csm_status csm_match_noalloc(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts *opts,
+                                        csm_result *out) {
+  if (!ctx || !g || !q || !out || q->graph_n != g->n) return CSM_E_ARG;
+  csm_match_opts o{};
+  o.refine = 1;
+  if (opts) o = *opts;
+  std::memset(out, 0, sizeof(*out));
+  if (o.list && o.list_capacity)
+    return set_error(ctx, CSM_E_ARG, "csm_match_noalloc counts only: the sorted list needs arena memory");
+  o.list = nullptr;
+  hipSetDevice(ctx->device);
+  void *region = scratch_region(ctx->device);
+  if (!region) return set_error(ctx, CSM_E_DEVICE, "static scratch not resolved on this device (csm_open first)");
+  WorkMem mem(ctx, region, kScratchBytes);
+  const bool profiling = ctx->profiling;  // no HIP events: they may allocate
+  ctx->profiling = false;
+  auto t0 = std::chrono::steady_clock::now();
+  csm_status st = CSM_OK;
+  if (!(o.reuse_filter && q->filtered)) st = run_filter(ctx, g, q, o.refine, &mem);
+  if (st == CSM_OK && o.reorder) st = apply_reorder(ctx, g, q);
+  if (st == CSM_OK) {
+    for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
+    out->ms_filter = q->ms_filter;
+    mem.release();  // the filter's scratch is not needed by the join
+    st = run_join(ctx, g, q, o, *out, &mem);
+  }
+  ctx->profiling = profiling;
+  if (st == CSM_E_NOMEM)
+    set_error(ctx, st, "workspace beyond the static scratch of " + std::to_string(kScratchBytes >> 20) +
+                           " MiB (" + ctx->last_error + ")");
   if (st != CSM_OK || out->status != CSM_E_TIMEOUT) out->status = st;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return st;

@@ -8,6 +8,35 @@
 
 namespace csm {
 
+// ---------------------------------------------------------- static scratch
+// Workspace of csm_match_noalloc: a file-scope device array, one copy per GPU
+// the library runs on, held until the process ends.  Host code never uses the
+// array's name as an address: its device address is resolved once per device
+// (hipGetSymbolAddress, with that device current, at csm_open) and kept here.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+csm_status scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return CSM_E_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return CSM_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CSM_E_DEVICE;
+  g_scratch_ptr[device] = p;
+  return CSM_OK;
+}
+}  // namespace
+
+void *scratch_region(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 // ------------------------------------------------------------------ arena
 csm_status Arena::init(uint64_t bytes) {
   std::lock_guard<std::mutex> g(mu_);
@@ -459,6 +488,7 @@ csm_status csm_open(int device, uint64_t arena_bytes, csm_ctx **out) {
   ctx->num_cus = static_cast<uint32_t>(prop.multiProcessorCount);
   ctx->arch = prop.gcnArchName;
   if (ctx->arch.rfind("gfx950", 0) != 0) return CSM_E_NO_GPU;  // built for gfx950 only
+  if (scratch_setup(device) != CSM_OK) return CSM_E_DEVICE;  // device current
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return CSM_E_DEVICE;
   if (hipHostMalloc(reinterpret_cast<void **>(&ctx->pinned), 4096, hipHostMallocDefault) != hipSuccess) {
     hipStreamDestroy(ctx->stream);

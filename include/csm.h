@@ -244,6 +244,21 @@ csm_status csm_copy_probe(csm_ctx *ctx, uint64_t bytes, uint32_t iters, double *
 csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q,
                      const csm_match_opts *opts, csm_result *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): csm_match in count-only mode (opts->list must be NULL)
+ * whose filter scratch, candidate index and join workspace all live in the
+ * library's static device scratch of CSM_NOALLOC_SCRATCH_BYTES per GPU,
+ * resolved at csm_open.  It calls no hipMalloc, hipFree or allocating library
+ * routine and takes no arena block.  A query whose workspace does not fit
+ * returns CSM_E_NOMEM with csm_last_error naming the limit; the context stays
+ * usable.  Every call on one GPU shares that GPU's scratch: calls that overlap
+ * on one GPU (from several contexts or threads) race and must be serialised by
+ * the caller.  Results equal csm_match's. */
+#define CSM_NOALLOC_SCRATCH_BYTES (128ull << 20)
+csm_status csm_match_noalloc(csm_ctx *ctx, const csm_graph *g, csm_query *q,
+                             const csm_match_opts *opts, csm_result *out);
+
 /* Candidate-aware matching order for q, from its filtered candidate counts
  * (the filter runs first if it has not): the root has the fewest candidates
  * per query degree, then each position takes the unmatched vertex with the

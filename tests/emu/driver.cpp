@@ -1,6 +1,7 @@
 // Runs the library, built for the CPU emulation, on graph files:
 //   driver <data.graph> <cap> <query.graph>...
-// prints per query "plain <count> <capped>", then its list under a cap of 64
+// prints per query "plain <count> <capped>", the same from csm_match_noalloc
+// ("noalloc ..."), then its list under a cap of 64
 // ("list <rows>" and one "row ..." line per embedding, sorted, query-id
 // order), "batch <count> <capped>", and with the candidate-aware order
 // (reorder option) "plainr ..." and "batchr ...".
@@ -45,6 +46,8 @@ int main(int argc, char **argv) {
     csm_result r{};
     check(csm_match(ctx, g, q, &o, &r), "match", ctx);
     std::printf("plain %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+    check(csm_match_noalloc(ctx, g, q, &o, &r), "match noalloc", ctx);
+    std::printf("noalloc %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
     const uint32_t nq = plans[i].num_v;
     std::vector<uint32_t> rows(64ull * nq);
     csm_match_opts lo{};

@@ -58,6 +58,12 @@ hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t st = nullptr);
 hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t st = nullptr);
 hipError_t hipMemset(void *d, int v, size_t n);
+// a device global's address: on the CPU the object itself
+template <typename T>
+inline hipError_t hipGetSymbolAddress(void **p, T *sym) {
+  *p = static_cast<void *>(sym);
+  return hipSuccess;
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned f);
 hipError_t hipStreamCreate(hipStream_t *s);
 hipError_t hipStreamDestroy(hipStream_t s);
