@@ -13,16 +13,18 @@
  *                                 random queries); invariants |E_T|=|V|-1
  *   vertex cover                  brute-force minimum cover size on random
  *                                 trees; worked example (cover [1,3])
- *   clustering / root / order     worked example trace of the bundled pair
- *                                 (compact [1,0,2,3,3,1,4,3,1], root 3,
- *                                 order (3,1),(3,4),(1,0),(1,2),(0,2)); the
- *                                 product planner (independent C++) agrees on
- *                                 thousands of random queries.  Only the
- *                                 worked example pins these against the
- *                                 reference itself; the rest is restatement.
+ *   clustering / root / order     seven traces made by hand from the
+ *                                 reference source (tests/golden/
+ *                                 planner_traces.json and the bundled pair:
+ *                                 merges through a core and through shared
+ *                                 members, combines that create a cluster or
+ *                                 keep a holder, several layers); the product
+ *                                 planner (independent C++) agrees with them
+ *                                 and on thousands of random queries
  *   oc_filter_nlc                 brute-force predicate per (u, v) in Python
- *   oc_refine                     soundness: every embedding found by brute
- *                                 force survives; subset of oc_filter_nlc
+ *   oc_refine                     the spec recomputed with Python sets
+ *                                 (exact); soundness against brute force;
+ *                                 the worked example's refined sets
  *   oc_match                      brute-force enumeration and networkx
  *                                 monomorphism goldens (tests/golden), worked
  *                                 example count 4 and its sorted list

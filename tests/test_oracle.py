@@ -144,6 +144,57 @@ def test_refinement_sound_and_subset():
                 assert t[u] in ref[u], (it, u, t)
 
 
+def _refine_by_spec(nbr, sets, clusters):
+    """The refinement spec (csm.h, csm_filter) with Python sets: for every
+    flattened cluster [k, m1, m2, ...] in order, first the core step
+    C(k) <- {v in C(k) : N(v) meets C(m) for every member m}, then the member
+    step C(m) <- {w in C(m) : N(w) meets C(k)} for each member."""
+    cur = [set(s) for s in sets]
+    for cl in clusters:
+        k, members = cl[0], cl[1:]
+        cur[k] = {v for v in cur[k] if all(nbr[v] & cur[m] for m in members)}
+        for m in members:
+            cur[m] = {w for w in cur[m] if nbr[w] & cur[k]}
+    return cur
+
+
+def test_refinement_equals_spec():
+    """oc_refine equals the spec computed with sets, exactly, starting from the
+    NLC sets; and the refinement removes candidates on these inputs, so a
+    refinement that did nothing (or skipped a step) would fail."""
+    r = random.Random(17)
+    removed = 0
+    for it in range(150):
+        n = r.randint(8, 40)
+        nl = r.randint(1, 3)
+        dl, de = rand_graph(r, n, r.uniform(0.05, 0.3), nl)
+        nq = r.randint(3, 9)
+        ql, qe = rand_query(r, nq, r.randint(0, 3), nl)
+        q = O.Query.from_arrays(ql, qe)
+        p = q.plan()
+        d = O.Data.from_arrays(dl, de)
+        nbr = [set() for _ in range(n)]
+        for a, b in de:
+            nbr[a].add(b)
+            nbr[b].add(a)
+        base = [set(map(int, s)) for s in O.bits_to_sets(O.filter_nlc(d, n, q), n)]
+        got = [set(map(int, s)) for s in O.bits_to_sets(O.refine(d, n, q, p), n)]
+        want = _refine_by_spec(nbr, base, p.clusters)
+        assert got == want, it
+        removed += sum(len(b) - len(w) for b, w in zip(base, want))
+    assert removed > 100, removed
+
+
+def test_refinement_spec_on_the_worked_example():
+    """SURVEY App. B's refined sets (u0,u2: {0,4,5,6,7,8}; u1: {1,6,8};
+    u3: {2,9}; u4: {3,10}) from the spec over the traced clusters."""
+    qp, gp = example_paths()
+    q = O.Query.read(qp)
+    d = O.Data.read(gp)
+    got = [sorted(map(int, s)) for s in O.bits_to_sets(O.refine(d, 11, q, q.plan()), 11)]
+    assert got == [[0, 4, 5, 6, 7, 8], [1, 6, 8], [0, 4, 5, 6, 7, 8], [2, 9], [3, 10]]
+
+
 def _brute_min_cover(n, tree_edges):
     for k in range(n + 1):
         for S in itertools.combinations(range(n), k):
