@@ -106,7 +106,7 @@ class Result(C.Structure):
     _fields_ = [("count", C.c_uint64), ("capped", C.c_uint32), ("status", C.c_uint32),
                 ("listed", C.c_uint64), ("partial_rows", C.c_uint64), ("items", C.c_uint64),
                 ("ms_filter", C.c_double), ("ms_join", C.c_double), ("ms_total", C.c_double),
-                ("ms_latency", C.c_double), ("path", C.c_uint32), ("reserved", C.c_uint32),
+                ("ms_latency", C.c_double), ("path", C.c_uint32), ("anomalies", C.c_uint32),
                 ("kernels", KernelStat * K_NUM)]
 
     def kernel(self, cls):

@@ -1425,6 +1425,7 @@ csm_status BatchRun::finish(csm_result *results) {
     r.items = hst[i].items;
     r.partial_rows = hst[i].rows;
     r.path = active_[i] ? CSM_PATH_BATCH : CSM_PATH_HOST;
+    r.anomalies = static_cast<uint32_t>(std::min<unsigned long long>(hctl.late_recs + hctl.bad_recs, 0xFFFFFFFFull));
     if (i == 0) {
       for (int c = 0; c < CSM_K_NUM; ++c) r.kernels[c] = kstat_[c];
       r.kernels[CSM_K_BATCH].launches = 1;

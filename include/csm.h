@@ -219,7 +219,9 @@ typedef struct csm_result {
   double ms_total;
   double ms_latency;       /* serving: arrival to result */
   uint32_t path;           /* csm_path: which method produced the result */
-  uint32_t reserved;
+  uint32_t anomalies;      /* batch paths: task records the persistent kernel read late or found bad in
+                              this call (never expected; any makes the call recount its unfinished
+                              queries on the plain path and is also reported on stderr) */
   csm_kernel_stat kernels[CSM_K_NUM];
 } csm_result;
 

@@ -45,7 +45,8 @@ def test_batch_small_random_vs_oracle(ctx):
                 print("batch round", round_, "queries", len(qs), "cap", cap, "graph", use_graph, flush=True)
                 # a short time limit turns a scheduling fault into a reported abort
                 res = csm.batch_match(g, qs, cap=cap, use_graph=use_graph, time_limit_ms=20000)
-                assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in res), [x.path for x in res]
+                assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) and x.anomalies == 0 for x in res), \
+                    [(x.path, x.anomalies) for x in res]
                 got = [(x.count, bool(x.capped)) for x in res]
                 bad = [(i, queries[i], got[i], want[i]) for i in range(len(got)) if got[i] != want[i]]
                 assert not bad, (round_, cap, use_graph, bad[:3])
@@ -67,7 +68,7 @@ def test_batch_equals_plain_and_alone(ctx):
     qs = [g.prepare(p) for p in plans]
     cap = 200_000
     mix = csm.batch_match(g, qs, cap=cap, time_limit_ms=20000)
-    assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in mix)
+    assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) and x.anomalies == 0 for x in mix)
     for i, q in enumerate(qs):
         plain, _ = q.match(cap=cap)
         alone = csm.batch_match(g, [q], cap=cap, time_limit_ms=20000)[0]
@@ -112,7 +113,7 @@ def test_batch_tuning_variants_agree(ctx):
         res = csm.batch_match(g, qs, cap=cap, time_limit_ms=20000, **kw)
         assert [(x.count, x.capped) for x in res] == want, kw
         if "item_budget" not in kw:
-            assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) for x in res), kw
+            assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) and x.anomalies == 0 for x in res), kw
     for q in qs:
         q.free()
     g.free()

@@ -223,23 +223,29 @@ def test_config1_er_full_list(ctx):
 
 
 def test_config2_rmat16_counts(ctx):
-    """R-MAT scale 16, 16 Zipf labels: two 8-vertex queries, tree-like and with 3 extra cycle edges."""
+    """R-MAT scale 16, 16 Zipf labels: two 8-vertex queries, tree-like and with
+    3 extra cycle edges.  Both have more than 1e10 embeddings, so the first
+    100,000 in the matching order (listed, sorted) must be the oracle's, with
+    the same capped flag.  The oracle builds its CSR from the generator's raw
+    edge list, not from the device's."""
     hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
     g = ctx.upload(hg)
     host = g.to_host()
-    off, adj, lab = g.download()
-    d = O.Data.from_arrays(lab, host.arrays()[1])
+    dl, de = hg.arrays()
+    d = O.Data.from_arrays(dl, de)
+    n_list = 100_000
     for mode, extra, seed in ((csm.QUERY_TREE, 0, 11), (csm.QUERY_EXTRA, 3, 12)):
         hq = csm.gen_query(host, 8, mode, seed, extra_edges=extra)
         ql, qe = hq.arrays()
         oq = O.Query.from_arrays(ql, qe)
-        op = oq.plan()
-        cap = 2_000_000
-        oc, ocap, _, _, _ = O.match(d, oq, op, cap=cap, threads=8)
+        oc, ocap, orows, _, secs = O.match(d, oq, oq.plan(), cap=n_list, want_list=True, list_cap=n_list,
+                                           refine=False)
         q = g.prepare(hq.plan())
-        res, _ = q.match(cap=cap)
-        print("config2", mode, res.count, bool(res.capped), flush=True)
-        assert (res.count, bool(res.capped)) == (oc, ocap)
+        res, rows = q.match(cap=n_list, want_list=True)
+        print("config2", mode, res.count, bool(res.capped), f"oracle {secs:.1f}s", flush=True)
+        assert (res.count, bool(res.capped)) == (oc, ocap) == (n_list, True)
+        want = np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, len(ql))
+        assert np.array_equal(rows, want)
         q.free()
     g.free()
 

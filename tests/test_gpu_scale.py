@@ -1,4 +1,4 @@
-"""The batched and served workloads at full size:
+"""The batched and served workloads at full size, against the CPU oracle:
 * R-MAT scale 20 (2^24 edge samples), 32 labels Zipf(1.0), random-walk
   queries of 4..32 vertices, cap 1e8;
 * a Chung-Lu power-law graph of 16M vertices and 256M edge samples (degree
@@ -8,16 +8,19 @@
 * R-MAT scale 22 (2^26 edge samples), 32 labels Zipf(1.0), random-walk
   queries of 4..32 vertices (geometric, mean 10), cap 1e8.
 
-At these sizes the CPU oracle cannot enumerate the queries, so parity rests
-on properties that hold at any size:
-* the plain per-query path and one batch call over all the queries give
-  identical (count, capped) for every query both finish, and a query run
-  alone in a batch gives the same result as in the mix;
-* every embedding the plain path lists (first 2000 per query) is one: labels
-  equal, ids distinct, every query edge an edge of the device CSR, rows
-  distinct and sorted.
-The plain path itself is checked against the oracle on the smaller
-workloads."""
+Parity with the oracle, whose CSR is built from the generator's raw edge
+list (not from the device's):
+* every query's first 100,000 embeddings in the matching order, listed by
+  the plain path (sorted, query-id order), equal the oracle's, with the same
+  count and capped flag; a query with fewer embeddings is compared in full;
+* a query that stays under the workload's cap on the plain path has its
+  exact count compared with the oracle's (16 threads, bounded time);
+* the batched path (one persistent kernel over all the queries) gives the
+  plain path's (count, capped) at the workload's cap, with no query recounted
+  on the plain path and no late or bad task record, and a query alone in a
+  batch gives what it gives in the mix.
+The oracle runs with a time budget per query; a query it cannot finish is
+reported and left out, and each test requires a floor of compared queries."""
 import random
 import time
 
@@ -25,10 +28,11 @@ import numpy as np
 import pytest
 
 import clustersm_amd as csm
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-CAP = 1_000_000_000
+N_LIST = 100_000
 
 
 @pytest.fixture(scope="module")
@@ -38,31 +42,55 @@ def ctx():
     c.close()
 
 
-def check_listed(g, qs, plans, n_list=2000):
-    """The plain path's first n_list embeddings of each query are embeddings
-    of the device CSR: labels equal, ids distinct, every query edge present,
-    rows distinct and sorted."""
-    off, adj, lab = g.download()
-    checked = 0
-    for i, (q, p) in enumerate(zip(qs, plans)):
-        res, rows = q.match(cap=n_list, want_list=True, time_limit_ms=3000)
-        if res.status != 0 or res.count == 0:
+def oracle_data(hg):
+    t0 = time.time()
+    labels, edges = hg.arrays()
+    d = O.Data.from_arrays(labels, edges)
+    print(f"oracle CSR in {time.time() - t0:.1f}s", flush=True)
+    return d
+
+
+def prefix_vs_oracle(d, items, gpu_limit_ms=5000, oracle_budget_s=20.0):
+    """items: (index, device query, host query).  Returns (compared, uncapped)."""
+    compared = uncapped = 0
+    for i, q, hq in items:
+        res, rows = q.match(cap=N_LIST, want_list=True, time_limit_ms=gpu_limit_ms)
+        if res.status != 0:
+            print("prefix", i, "plain path over its time limit", flush=True)
             continue
-        rows = rows[: min(res.count, n_list)]
-        print("list", i, len(rows), flush=True)
-        assert len(rows) == min(res.count, n_list)
-        assert all((lab[rows[:, u]] == p.labels[u]).all() for u in range(p.num_v)), i
-        srt = np.sort(rows, axis=1)
-        assert (srt[:, 1:] != srt[:, :-1]).all(), i
-        for e in range(p.num_e):
-            for x, y in zip(rows[:, p.edge_src[e]].tolist(), rows[:, p.edge_dst[e]].tolist()):
-                lo, hi = int(off[x]), int(off[x + 1])
-                k = lo + int(np.searchsorted(adj[lo:hi], y))
-                assert k < hi and adj[k] == y, (i, e, x, y)
-        keys = [tuple(r) for r in rows.tolist()]
-        assert keys == sorted(set(keys)), i
-        checked += 1
-    return checked
+        ql, qe = hq.arrays()
+        oq = O.Query.from_arrays(ql, qe)
+        c, capped, orows, timed_out, secs = O.match(d, oq, oq.plan(), cap=N_LIST, want_list=True, list_cap=N_LIST,
+                                                    budget_s=oracle_budget_s, refine=False)
+        print("prefix", i, res.count, bool(res.capped), "oracle", c, capped, f"{secs:.1f}s",
+              "timed out" if timed_out else "", flush=True)
+        if timed_out:
+            continue
+        assert (res.count, bool(res.capped)) == (c, capped), i
+        want = np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, len(ql))
+        assert np.array_equal(rows, want), i
+        compared += 1
+        uncapped += not capped
+    return compared, uncapped
+
+
+def counts_vs_oracle(d, items, plain, cap, budget_s=20.0):
+    """Exact counts of the queries the plain path finds under `cap`."""
+    compared = 0
+    for (i, q, hq), p in zip(items, plain):
+        if p.status != 0 or p.capped:
+            continue
+        ql, qe = hq.arrays()
+        oq = O.Query.from_arrays(ql, qe)
+        c, capped, _, timed_out, secs = O.match(d, oq, oq.plan(), cap=cap, threads=16, budget_s=budget_s,
+                                                refine=False)
+        print("count", i, p.count, "oracle", c, capped, f"{secs:.1f}s", "timed out" if timed_out else "",
+              flush=True)
+        if timed_out:
+            continue
+        assert (p.count, False) == (c, capped), i
+        compared += 1
+    return compared
 
 
 def batch_equals_plain(g, qs, cap, plain_limit_ms):
@@ -72,87 +100,91 @@ def batch_equals_plain(g, qs, cap, plain_limit_ms):
         print("plain", i, res.status, res.count, res.capped, f"{res.ms_total:.0f} ms", flush=True)
         plain.append(res)
     batch = csm.batch_match(g, qs, cap=cap, time_limit_ms=30000)
+    assert all(x.path != csm.PATH_BATCH_ABORT and x.anomalies == 0 for x in batch), \
+        [(x.path, x.anomalies) for x in batch]
     both = [i for i in range(len(qs)) if plain[i].status == 0 and batch[i].status == 0]
     for i in both:
         assert (batch[i].count, batch[i].capped) == (plain[i].count, plain[i].capped), i
-    print("compared", len(both), "of", len(qs), "; uncapped", sum(1 for i in both if not plain[i].capped), flush=True)
+    print("batch == plain on", len(both), "of", len(qs), flush=True)
     return plain, batch, both
 
 
-def test_rmat20_batch_equals_plain(ctx):
+def test_rmat20_vs_oracle(ctx):
     hg = csm.gen_rmat(20, 1 << 24, 32, 1.0, 3)
     g = ctx.upload(hg)
     host = g.to_host()
     r = random.Random(5)
-    plans = []
+    hqs = []
     for i in range(32):
         nv, avg = r.randint(4, 32), r.uniform(2, 4)
-        plans.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 1000 + i, avg_degree=avg).plan())
-    qs = [g.prepare(p) for p in plans]
-    _, _, both = batch_equals_plain(g, qs, 100_000_000, 2000)
+        hqs.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 1000 + i, avg_degree=avg))
+    qs = [g.prepare(h.plan()) for h in hqs]
+    items = list(zip(range(32), qs, hqs))
+    plain, _, both = batch_equals_plain(g, qs, 100_000_000, 2000)
     assert len(both) >= 28
-    assert check_listed(g, qs[:8], plans[:8]) >= 4
+    d = oracle_data(hg)
+    compared, uncapped = prefix_vs_oracle(d, items)
+    exact = counts_vs_oracle(d, items, plain, 100_000_000, budget_s=90.0)
+    print("rmat20: prefixes compared", compared, "uncapped", uncapped, "exact counts", exact, flush=True)
+    assert compared >= 24 and exact >= 1
     for q in qs:
         q.free()
     g.free()
 
 
-def test_rmat22_batch_equals_plain(ctx):
+def test_rmat22_vs_oracle(ctx):
     hg = csm.gen_rmat(22, 1 << 26, 32, 1.0, 11)
     g = ctx.upload(hg)
     host = g.to_host()
     r = random.Random(7)
-    plans = []
+    hqs = []
     for i in range(16):
         nv = 4
         while nv < 32 and r.random() > 1.0 / 7.0:
             nv += 1
-        plans.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 7000 + i, avg_degree=r.uniform(2, 4)).plan())
-    qs = [g.prepare(p) for p in plans]
-    _, _, both = batch_equals_plain(g, qs, 100_000_000, 2000)
+        hqs.append(csm.gen_query(host, nv, csm.QUERY_AVG_DEGREE, 7000 + i, avg_degree=r.uniform(2, 4)))
+    qs = [g.prepare(h.plan()) for h in hqs]
+    items = list(zip(range(16), qs, hqs))
+    plain, _, both = batch_equals_plain(g, qs, 100_000_000, 2000)
     assert len(both) >= 12
-    assert check_listed(g, qs[:6], plans[:6]) >= 3
+    d = oracle_data(hg)
+    compared, uncapped = prefix_vs_oracle(d, items)
+    exact = counts_vs_oracle(d, items, plain, 100_000_000, budget_s=60.0)
+    print("rmat22: prefixes compared", compared, "uncapped", uncapped, "exact counts", exact, flush=True)
+    assert compared >= 12
     for q in qs:
         q.free()
     g.free()
 
 
-def test_chung_lu_16m_batch_equals_plain(ctx):
+def test_chung_lu_16m_vs_oracle(ctx):
     t0 = time.time()
     hg = csm.gen_chung_lu(16_000_000, 256_000_000, 2.1, 50_000, 64, 0.8, 41)
     g = ctx.upload(hg)
-    del hg
     n, m, maxdeg = g.info()
     print(f"graph |V|={n} |E|={m} max degree {maxdeg} in {time.time() - t0:.1f}s", flush=True)
     assert n == 16_000_000 and m > 200_000_000 and maxdeg <= 50_000
     host = g.to_host()
     r = random.Random(4)
-    plans = []
+    hqs = []
     for i in range(8):
         nv, extra = r.randint(12, 24), r.randint(2, 6)
-        plans.append(csm.gen_query(host, nv, csm.QUERY_EXTRA, 300 + i, extra_edges=extra).plan())
+        hqs.append(csm.gen_query(host, nv, csm.QUERY_EXTRA, 300 + i, extra_edges=extra))
         print("query", i, "vertices", nv, "extra edges", extra, flush=True)
     del host
-    qs = [g.prepare(p) for p in plans]
-    plain = []
-    for i, q in enumerate(qs):
-        res, _ = q.match(cap=CAP, time_limit_ms=3000)
-        print("plain", i, res.status, res.count, res.capped, f"{res.ms_total:.0f} ms", flush=True)
-        plain.append(res)
-    batch = csm.batch_match(g, qs, cap=CAP, time_limit_ms=30000)
-    for i, x in enumerate(batch):
-        print("batch", i, x.status, x.count, x.capped, x.path, flush=True)
-    both = [i for i in range(len(qs)) if plain[i].status == 0 and batch[i].status == 0]
-    assert len(both) >= len(qs) // 2, (both, [p.status for p in plain], [x.status for x in batch])
-    for i in both:
-        assert (batch[i].count, batch[i].capped) == (plain[i].count, plain[i].capped), i
-    # alone in a batch: the same as in the mix
+    qs = [g.prepare(h.plan()) for h in hqs]
+    cap = 1_000_000_000
+    plain, batch, both = batch_equals_plain(g, qs, cap, 3000)
+    assert len(both) >= len(qs) // 2
     j = both[0]
-    alone = csm.batch_match(g, [qs[j]], cap=CAP, time_limit_ms=30000)[0]
+    alone = csm.batch_match(g, [qs[j]], cap=cap, time_limit_ms=30000)[0]
     print("alone", j, alone.status, alone.count, alone.capped, flush=True)
     assert alone.status == 0 and (alone.count, alone.capped) == (batch[j].count, batch[j].capped)
-    # listed embeddings are embeddings
-    assert check_listed(g, qs, plans) == len(qs)
+    d = oracle_data(hg)
+    del hg
+    compared, uncapped = prefix_vs_oracle(d, list(zip(range(8), qs, hqs)), oracle_budget_s=30.0)
+    print("chung-lu: prefixes compared", compared, "uncapped", uncapped, flush=True)
+    assert compared >= 6
     for q in qs:
         q.free()
     g.free()

@@ -54,6 +54,7 @@ def test_serve_results_equal_alone(ctx):
         assert res.status == 0, (i, res.status)
         assert res.ms_latency > 0
         assert res.path in (csm.PATH_BATCH, csm.PATH_HOST, csm.PATH_BATCH_BUDGET)
+        assert res.anomalies == 0
         got.append((res.count, res.capped))
     ctx.serve_stop()
     print("served", len(got), "queries; capped", sum(c for _, c in got), flush=True)
