@@ -1,37 +1,45 @@
 """Benchmark of the MI355X matching path; prints one JSON line.
 
-Headline (value): embeddings/s of the plain per-query path on R-MAT scale 16
-(65,536 vertices, 2^20 edge samples, a=0.57 b=0.19 c=0.19, loops and
-duplicates removed), 16 labels drawn Zipf(1.0); two 8-vertex random-walk
-queries, one tree-like and one with 3 extra cycle edges.  One step = filter
-+ refinement + join of both queries with the graph resident in HBM.  Both
-queries have more than 1e10 embeddings, so each is enumerated up to a
-per-query cap (count-only); value = embeddings counted / second.  Kernel
-times come from HIP events recorded on the launch stream during the timed
-steps; the roofline entry is the kernel class with the most device time.
-`reordered` times the same steps with the candidate-aware matching order
-(the reorder option of csm_match), which keeps every count.
+Headline (value): embeddings/s of the batched path on the large cyclic
+workload: a Chung-Lu power-law graph of 16,000,000 vertices from 256M edge
+samples (degree exponent 2.1, maximum degree 50,000; loops and duplicates
+removed), 64 labels drawn Zipf(0.8), resident in HBM; 128 random-walk
+queries of 12..24 vertices with 2..6 extra (non-tree) edges each,
+count-only with a cap of 1e9 embeddings per query.  One step = one
+csm_batch_match (filter, candidate indexes and one persistent join kernel)
+over a group of 16 of the 128 queries, the groups taken in turn; value =
+embeddings counted in the timed steps / their time.  The same 128 queries
+are also run once per query on the plain path (csm_match) and once in a
+single batch call, for queries/s and every count compared.  The roofline
+entry is the persistent join kernel, timed with HIP events on its stream:
+algorithmic bytes = the reference's extension pass over the same rows
+(join.cu:99-177: row, offsets, and id + candidate word per adjacency entry
+of the row's parent vertex, rows written), with the kernel's own logical
+bytes beside it.
 
-Batched: R-MAT scale 20 (2^24 edge samples), 32 labels Zipf(1.0); 256
+c2_full: the plain path on R-MAT scale 16 (65,536 vertices, 2^20 edge
+samples, a=0.57 b=0.19 c=0.19), 16 labels Zipf(1.0), two 8-vertex
+random-walk queries (tree-like; with 3 extra cycle edges), full enumeration
+(no cap) under a time limit per query: an unfinished query reports the
+count reached as a lower bound.
+
+batched: R-MAT scale 20 (2^24 edge samples), 32 labels Zipf(1.0); 256
 random-walk queries of 4..32 vertices with average degree 2..4, per-query
-cap 1e8.  The plain path runs them query by query with a 2 s limit each; a
-few of these queries have no embedding after 1e11 candidate tests and are
-listed and left out.  One csm_batch_match call (one persistent kernel, no
-time limit, exact) runs every other query; queries/s is compared on that
-same set and every count must equal the plain path's.  `reordered` repeats
-the comparison with the candidate-aware matching order on both paths.
+cap 1e8, in the reference's matching order.  The plain path runs them query
+by query and one csm_batch_match runs all 256, both with the same 2 s limit
+per query (a query over it reports a lower bound on both paths); queries/s
+compared over all 256 and every finished count must agree.  `reordered`
+repeats the comparison with the candidate-aware matching order.
 
-Served: R-MAT scale 22 (2^26 edge samples), 32 labels Zipf(1.0), resident;
-128 random-walk queries (4..32 vertices, geometric with mean 10) arriving as
-a Poisson stream at 32 queries/s (and, in other_loads, at 64 and 96).  A
-batched server admits every request that arrived while the previous batch
-ran (footprint estimate within 64 GB, at most 64) into one csm_batch_match;
-p50/p99 latency from arrival to result beside a FIFO plain per-query server
-over the same arrivals.
+served: R-MAT scale 22 (2^26 edge samples), 32 labels Zipf(1.0), resident;
+128 random-walk queries (4..32 vertices, geometric with mean 10) arriving
+as Poisson streams at 32, 64 and 96 queries/s; p50/p99 latency from arrival
+to result of the servers beside a FIFO plain per-query server.
 
 cpu_baseline times the CPU oracle (same graph, queries, matching order and
-cap) on a bounded sample of the headline workload, with up to 16 threads and
-single-threaded, and names the host CPU.
+cap) on a bounded sample of the headline workload (first queries of the
+first group, a time budget each), with up to 16 threads, and names the host
+CPU.
 """
 import argparse
 import json
@@ -51,16 +59,24 @@ METRIC = "embeddings/s + batched queries/s vs per-query launch; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 
 
-def build_workload():
-    hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
-    return hg, [
-        ("tree", csm.gen_query(hg, 8, csm.QUERY_TREE, 11)),
-        ("cycles+3", csm.gen_query(hg, 8, csm.QUERY_EXTRA, 12, extra_edges=3)),
-    ]
+C4_QUERIES = 128
+C4_CAP = 1_000_000_000
 
 
-def run_step(queries, cap, reorder=False):
-    return [q.match(cap=cap, reorder=reorder)[0] for q in queries]
+def c4_workload():
+    """Chung-Lu 16M vertices / 256M edge samples, 64 Zipf(0.8) labels, and its
+    128 random-walk queries (12..24 vertices, 2..6 extra edges)."""
+    hg = csm.gen_chung_lu(16_000_000, 256_000_000, 2.1, 50_000, 64, 0.8, 41)
+    return hg
+
+
+def c4_queries(host, n):
+    r = random.Random(4)
+    out = []
+    for i in range(n):
+        nv, extra = r.randint(12, 24), r.randint(2, 6)
+        out.append(csm.gen_query(host, nv, csm.QUERY_EXTRA, 300 + i, extra_edges=extra))
+    return out
 
 
 def kernel_totals(result_lists):
@@ -69,18 +85,19 @@ def kernel_totals(result_lists):
         for r in results:
             for c in range(csm.K_NUM):
                 k = r.kernels[c]
-                t = tot.setdefault(c, [0, 0, 0])
+                t = tot.setdefault(c, [0, 0, 0, 0])
                 t[0] += k.launches
                 t[1] += k.ns
                 t[2] += k.bytes
+                t[3] += k.bytes_ref
     return tot
 
 
-def measured_traffic():
-    """HBM bytes per launch by kernel class, from the committed rocprofv3 PMC
+def measured_traffic(profile):
+    """HBM bytes per launch by kernel class, from a committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, calibrated with a copy kernel)."""
     out = {}
-    path = os.path.join(ROOT, "profiles", "count_kernel_rmat16_plain.md")
+    path = os.path.join(ROOT, "profiles", profile)
     if os.path.exists(path):
         for ln in open(path):
             if ln.startswith("traffic_bytes_per_launch:"):
@@ -89,12 +106,16 @@ def measured_traffic():
     return out
 
 
-def roofline_from(tot):
-    """Dominant kernel class: achieved = algorithmic bytes / device time (bytes/ns = GB/s)."""
-    dom = max(tot, key=lambda c: tot[c][1])
-    launches, ns, byts = tot[dom]
+def roofline_from(tot, profile, cls=None):
+    """Dominant kernel class (or `cls`): achieved = algorithmic bytes (the
+    kernel's logical reads and writes) / device time (bytes/ns = GB/s);
+    ref_equivalent = the reference algorithm's bytes for the same rows / time,
+    which exceeds any bandwidth where the candidate index skips adjacency
+    scans (not a roofline figure)."""
+    dom = cls if cls is not None else max(tot, key=lambda c: tot[c][1])
+    launches, ns, byts, ref = tot[dom]
     achieved = byts / ns if ns else 0.0
-    traffic = measured_traffic().get(csm.KERNEL_NAMES[dom])
+    traffic = measured_traffic(profile).get(csm.KERNEL_NAMES[dom])
     return {
         "kernel": csm.KERNEL_NAMES[dom],
         "bound": "hbm",
@@ -106,11 +127,14 @@ def roofline_from(tot):
         "launches": launches,
         "avg_launch_us": round(ns / launches / 1e3, 2) if launches else None,
         "bytes_per_launch": byts // launches if launches else None,
+        "ref_bytes_per_launch": ref // launches if launches else None,
+        "ref_equivalent_GBps": round(ref / ns, 1) if ns else None,
     }
 
 
 def kernels_json(tot):
-    return {csm.KERNEL_NAMES[c]: {"launches": v[0], "ms": round(v[1] / 1e6, 3), "GB": round(v[2] / 1e9, 3)}
+    return {csm.KERNEL_NAMES[c]: {"launches": v[0], "ms": round(v[1] / 1e6, 3), "GB": round(v[2] / 1e9, 3),
+                                  "ref_GB": round(v[3] / 1e9, 3)}
             for c, v in tot.items() if v[0]}
 
 
@@ -125,49 +149,153 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(hg, host_queries, cap, seconds, single_seconds=6.0):
+def cpu_baseline(hg, host_queries, cap, seconds):
     """The CPU backtracking matcher (same matching order, filter and cap) on
-    the same graph and queries, each query stopped by a time budget: with up
-    to 16 threads over root candidates, then single-threaded."""
+    the same graph and queries, each query stopped by a time budget, with up
+    to 16 threads over root candidates."""
     from oracle import oracle as O
 
+    t0 = time.time()
     labels, edges = hg.arrays()
     data = O.Data.from_arrays(labels, edges)
-
-    def run(threads, budget):
-        total, secs, timed_out = 0, 0.0, False
-        per = budget / len(host_queries)
-        for hq in host_queries:
-            ql, qe = hq.arrays()
-            oq = O.Query.from_arrays(ql, qe)
-            c, capped, _, to, s = O.match(data, oq, oq.plan(), cap=cap, threads=threads, budget_s=per)
-            total += c
-            secs += s
-            timed_out |= to
-        return total, secs, timed_out, per
-
+    del labels, edges
+    build_s = time.time() - t0
     threads = min(os.cpu_count() or 1, 16)
-    total, secs, timed_out, per = run(threads, seconds)
-    t1, s1, to1, per1 = run(1, single_seconds)
+    per = seconds / len(host_queries)
+    total, secs, timed_out, capped = 0, 0.0, 0, 0
+    for hq in host_queries:
+        ql, qe = hq.arrays()
+        oq = O.Query.from_arrays(ql, qe)
+        c, cp, _, to, sec = O.match(data, oq, oq.plan(), cap=cap, threads=threads, budget_s=per, refine=False)
+        total += c
+        secs += sec
+        timed_out += to
+        capped += cp
     return {
         "value": round(total / secs, 1) if secs else 0.0,
         "unit": "embeddings/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"same graph and queries, backtracking oracle with {threads} threads, "
-                  f"{per:.0f} s budget per query ({'stopped by the budget' if timed_out else 'to the cap'}), "
-                  f"{total} embeddings in {secs:.1f} s",
-        "single_thread": {"value": round(t1 / s1, 1) if s1 else 0.0, "unit": "embeddings/s",
-                          "sample": f"same queries, 1 thread, {per1:.0f} s budget per query "
-                                    f"({'stopped by the budget' if to1 else 'to the cap'}), {t1} embeddings in {s1:.1f} s"},
+        "sample": f"same graph and cap, first {len(host_queries)} queries of the first group, backtracking oracle "
+                  f"with {threads} threads, {per:.1f} s budget per query ({timed_out} stopped by the budget, "
+                  f"{capped} reached the cap): {total} embeddings in {secs:.1f} s (oracle CSR build "
+                  f"{build_s:.1f} s, not timed)",
         "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
     }
 
 
-def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0, sample=0):
-    """The C3-shaped query set: the plain path query by query (each with a time
-    limit), then one batch call over every query the plain path finished,
-    exact (no limit); queries/s compared on that same set.
+def c4_section(ctx, args):
+    """The headline: timed steps of batched groups, then one pass of the plain
+    path and one whole-set batch call over the same 128 queries."""
+    t0 = time.time()
+    hg = c4_workload()
+    g = ctx.upload(hg)
+    host = g.to_host()
+    hqs = c4_queries(host, C4_QUERIES)
+    del host
+    qs = [g.prepare(h.plan()) for h in hqs]
+    nv, ne, maxdeg = g.info()
+    print(f"# C4 setup {time.time() - t0:.1f}s: |V|={nv} |E|={ne} max_degree={maxdeg}", file=sys.stderr, flush=True)
+    group = args.group
+    groups = [qs[i:i + group] for i in range(0, len(qs), group)]
+
+    def step(i):
+        return csm.batch_match(g, groups[i % len(groups)], cap=C4_CAP)
+
+    for i in range(args.warmup):
+        step(i)
+    ctx.set_profiling(True)
+    t1 = time.perf_counter()
+    steps = [step(args.warmup + i) for i in range(args.steps)]
+    elapsed = time.perf_counter() - t1
+    ctx.set_profiling(False)
+    embeddings = sum(x.count for st in steps for x in st)
+    anomalies = sum(x.anomalies for st in steps for x in st)
+    recounted = sum(1 for st in steps for x in st if x.path in (csm.PATH_BATCH_BUDGET, csm.PATH_BATCH_ABORT))
+    print(f"# C4 timed {args.steps} steps in {elapsed:.2f}s: {embeddings} embeddings", file=sys.stderr, flush=True)
+    tot = kernel_totals([[st[0]] for st in steps])  # a batch call's kernel totals are on its first result
+    # one pass of the plain path, and one batch call over all 128
+    plain, t_plain = [], 0.0
+    for q in qs:
+        t2 = time.perf_counter()
+        plain.append(q.match(cap=C4_CAP)[0])
+        t_plain += time.perf_counter() - t2
+    print(f"# C4 plain path: {len(qs)} queries in {t_plain:.2f}s", file=sys.stderr, flush=True)
+    t3 = time.perf_counter()
+    whole = csm.batch_match(g, qs, cap=C4_CAP)
+    t_whole = time.perf_counter() - t3
+    print(f"# C4 one batch call: {len(qs)} queries in {t_whole:.2f}s", file=sys.stderr, flush=True)
+    mism = [i for i, (x, p) in enumerate(zip(whole, plain)) if (x.count, x.capped, x.status) != (p.count, p.capped, 0)]
+    by_step = {}
+    for i, st in enumerate(steps):
+        base = ((args.warmup + i) % len(groups)) * group
+        for j, x in enumerate(st):
+            p = plain[base + j]
+            if (x.count, x.capped, x.status) != (p.count, p.capped, 0):
+                by_step.setdefault(base + j, [x.count, int(x.capped), x.status, p.count, int(p.capped)])
+    out = {
+        "elapsed": elapsed,
+        "embeddings": embeddings,
+        "tot": tot,
+        "graph": (nv, ne, maxdeg),
+        "detail": {
+            "queries_per_s": round(args.steps * group / elapsed, 2),
+            "plain_queries_per_s": round(len(qs) / t_plain, 2),
+            "plain_embeddings_per_s": round(sum(p.count for p in plain) / t_plain, 1),
+            "one_call_queries_per_s": round(len(qs) / t_whole, 2),
+            "one_call_embeddings_per_s": round(sum(x.count for x in whole) / t_whole, 1),
+            "speedup_vs_plain": round((args.steps * group / elapsed) / (len(qs) / t_plain), 2),
+            "one_call_speedup_vs_plain": round(t_plain / t_whole, 2),
+            "capped_queries": sum(1 for p in plain if p.capped),
+            "counts_equal_plain": not mism and not by_step,
+            "mismatches": mism[:8] + [[k] + v for k, v in list(by_step.items())[:8]],
+            "batch_anomalies": anomalies,
+            "recounted_on_plain_path": recounted,
+        },
+        "hg": hg,
+        "hqs": hqs,
+    }
+    for q in qs:
+        q.free()
+    g.free()
+    return out
+
+
+def c2_section(ctx, limits_ms):
+    """The plain path on R-MAT 16 with the two 8-vertex queries, no cap."""
+    hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
+    g = ctx.upload(hg)
+    out = []
+    named = [("tree", csm.gen_query(hg, 8, csm.QUERY_TREE, 11)),
+             ("cycles+3", csm.gen_query(hg, 8, csm.QUERY_EXTRA, 12, extra_edges=3))]
+    for (name, hq), lim in zip(named, limits_ms):
+        q = g.prepare(hq.plan())
+        ctx.set_profiling(True)
+        t = time.perf_counter()
+        r, _ = q.match(cap=0, time_limit_ms=lim)
+        dt = time.perf_counter() - t
+        ctx.set_profiling(False)
+        done = r.status == 0
+        print(f"# C2 {name}: {'exact' if done else 'lower bound'} {r.count} in {dt:.2f}s", file=sys.stderr,
+              flush=True)
+        tot = kernel_totals([[r]])
+        out.append({"name": name, "count": r.count, "exact": done, "seconds": round(dt, 3),
+                    "time_limit_ms": lim, "embeddings_per_s": round(r.count / dt, 1), "items": r.items,
+                    "roofline": roofline_from(tot, "count_kernel_rmat16_plain.md")})
+        q.free()
+    g.free()
+    return {"workload": "plain path, R-MAT scale 16 (2^20 samples, 16 Zipf(1.0) labels), two 8-vertex random-walk "
+                        "queries (tree-like; +3 cycle edges), full enumeration (no cap) under a time limit; an "
+                        "unfinished query's count is a lower bound",
+            "queries": out}
+
+
+def batched_section(ctx, nqueries, cap, item_budget, sample=0):
+    """The C3-shaped query set over all its queries, in the reference's
+    matching order: the plain path query by query, then one batch call, both
+    with the same per-query item budget (a query over it stops with a lower
+    bound on both paths); queries/s over every query, and every count that
+    both paths finish must agree.
 
     sample > 0 is for counter profiles: only the batch, with default options,
     over the first `sample` queries (warm-up call and timed call), with no
@@ -202,91 +330,109 @@ def batched_section(ctx, nqueries, cap, limit_ms, item_budget=0, sample=0):
     plain, plain_s = [], []
     for q in qs:
         t1 = time.perf_counter()
-        plain.append(q.match(cap=cap, time_limit_ms=limit_ms)[0])
+        plain.append(q.match(cap=cap, item_budget=item_budget)[0])
         plain_s.append(time.perf_counter() - t1)
-    keep = [i for i in range(nqueries) if plain[i].status == 0]
-    dropped = [i for i in range(nqueries) if plain[i].status != 0]
-    t_plain = sum(plain_s[i] for i in keep)
-    print(f"# plain path: {len(keep)} of {nqueries} queries finished within {limit_ms} ms each, "
-          f"{t_plain:.2f}s for those", file=sys.stderr, flush=True)
-    kq = [qs[i] for i in keep]
+    t_plain = sum(plain_s)
+    p_budget = [i for i in range(nqueries) if plain[i].status == csm.E_BUDGET]
+    print(f"# plain path: {nqueries} queries in {t_plain:.2f}s, {len(p_budget)} stopped by the item budget",
+          file=sys.stderr, flush=True)
     t3 = time.perf_counter()
-    tuned = csm.batch_autotune(g, kq, cap=cap, sample=64)  # untimed: a one-off per graph and query mix
-    if item_budget:
-        tuned.item_budget = item_budget
+    tuned = csm.batch_autotune(g, qs, cap=cap, sample=64, item_budget=item_budget)  # untimed: once per graph and mix
     tune_s = time.perf_counter() - t3
     print(f"# autotune {tune_s:.1f}s: wgs_per_cu={tuned.wgs_per_cu} max_out={tuned.max_out} "
           f"block_kb={tuned.block_kb} ilp={tuned.ilp}", file=sys.stderr, flush=True)
-    csm.batch_match(g, kq, opts=tuned)  # warm-up (code objects, arena layout)
+    csm.batch_match(g, qs, opts=tuned)  # warm-up (code objects, arena layout)
     ctx.set_profiling(True)
     t2 = time.perf_counter()
-    res = csm.batch_match(g, kq, opts=tuned)
+    res = csm.batch_match(g, qs, opts=tuned)
     t_batch = time.perf_counter() - t2
     ctx.set_profiling(False)
-    print(f"# batched: {len(keep)} queries in {t_batch:.2f}s", file=sys.stderr, flush=True)
-    mism = [[i, x.count, int(x.capped), x.status, x.path, plain[i].count, int(plain[i].capped)]
-            for x, i in zip(res, keep) if (x.count, x.capped, x.status) != (plain[i].count, plain[i].capped, 0)]
-    equal = not mism
+    b_budget = [i for i in range(nqueries) if res[i].status == csm.E_BUDGET]
+    print(f"# batched: {nqueries} queries in {t_batch:.2f}s, {len(b_budget)} stopped by the item budget",
+          file=sys.stderr, flush=True)
+    both = [i for i in range(nqueries) if plain[i].status == 0 and res[i].status == 0]
+    # finished on both: identical; stopped on one path: its count is a lower
+    # bound of the other's (checked exactly below with the reordered plain path)
+    mism = [[i, res[i].count, int(res[i].capped), res[i].status, res[i].path, plain[i].count, int(plain[i].capped),
+             plain[i].status]
+            for i in range(nqueries)
+            if (i in both and (res[i].count, res[i].capped) != (plain[i].count, plain[i].capped))
+            or (plain[i].status == csm.E_BUDGET and res[i].status == 0 and res[i].count < plain[i].count)
+            or (res[i].status == csm.E_BUDGET and plain[i].status == 0 and plain[i].count < res[i].count)]
     paths = {}
     for x in res:
         paths[x.path] = paths.get(x.path, 0) + 1
     tot = kernel_totals([[res[0]]])
-    bk = tot.get(csm.K_BATCH, [0, 0, 0])
+    bk = tot.get(csm.K_BATCH, [0, 0, 0, 0])
     out = {
         "workload": f"R-MAT scale 20 (2^24 samples, 32 Zipf(1.0) labels), {nqueries} random-walk queries of "
-                    f"4..32 vertices, average degree 2..4, cap {cap:.0e} per query, count-only",
-        "queries": len(keep),
-        "queries_per_s": round(len(keep) / t_batch, 2),
-        "plain_queries_per_s": round(len(keep) / t_plain, 2),
+                    f"4..32 vertices, average degree 2..4, cap {cap:.0e} per query, count-only, reference "
+                    f"matching order; both paths bound each query by {item_budget:.0e} items",
+        "queries": nqueries,
+        "queries_per_s": round(nqueries / t_batch, 2),
+        "plain_queries_per_s": round(nqueries / t_plain, 2),
         "speedup_vs_plain": round(t_plain / t_batch, 2),
-        "counts_equal_plain": equal,
+        "stopped_by_item_budget": {"plain": p_budget, "batch": b_budget,
+                                   "rule": "a query over the budget reports a lower bound; both paths stop the "
+                                           "same queries (none of them has an embedding in the plain path's "
+                                           "first items)"},
+        "counts_equal_plain": not mism,
+        "compared": len(both),
         "mismatches": mism,
         "autotuned": {"wgs_per_cu": tuned.wgs_per_cu, "max_out": tuned.max_out, "block_kb": tuned.block_kb,
-                      "ilp": tuned.ilp, "sample_queries": min(64, len(keep)), "tuning_s": round(tune_s, 1)},
-        "excluded": {"queries": dropped, "rule": f"the plain path does not finish them within {limit_ms} ms "
-                                                 f"(more than 1e11 candidate tests; the batch is not run on them)",
-                     "plain_items": [plain[i].items for i in dropped]},
+                      "ilp": tuned.ilp, "sample_queries": min(64, nqueries), "tuning_s": round(tune_s, 1)},
         "capped_queries": sum(1 for x in res if x.capped),
         "embeddings_per_s": round(sum(x.count for x in res) / t_batch, 1),
         "finished_by": {{csm.PATH_PLAIN: "plain", csm.PATH_HOST: "host", csm.PATH_BATCH: "batch kernel",
                          csm.PATH_BATCH_BUDGET: "plain after the item budget",
                          csm.PATH_BATCH_ABORT: "plain after an early stop"}[k]: v for k, v in paths.items()},
+        "batch_anomalies": sum(x.anomalies for x in res),
         "kernels": kernels_json(tot),
         "join_kernel": {"ms": round(bk[1] / 1e6, 1), "algorithmic_GB": round(bk[2] / 1e9, 2),
-                        "GB_per_s": round(bk[2] / bk[1], 1) if bk[1] else None},
+                        "GB_per_s": round(bk[2] / bk[1], 1) if bk[1] else None,
+                        "ref_GB": round(bk[3] / 1e9, 2)},
         # candidate tests (row x driver entry) done by each path; capped
         # queries overshoot their cap by different amounts on the two paths
-        "items": {"batch": sum(x.items for x in res), "plain": sum(plain[i].items for i in keep),
+        "items": {"batch": sum(x.items for x in res), "plain": sum(p.items for p in plain),
                   "top_batch": [[i, x.items, plain[i].items, int(x.capped)]
-                                for x, i in sorted(zip(res, keep), key=lambda t: -t[0].items)[:8]]},
+                                for i, x in sorted(enumerate(res), key=lambda t: -t[1].items)[:8]]},
     }
     # the same comparison with the candidate-aware matching order on both
     # paths (the reorder option: each path orders from its own filter counts,
     # inside the timed calls); counts and cap flags do not depend on the
     # order, so they must equal the default order's
-    plain_r, plain_rs = [], []
+    plain_r, t_plain_r = [], 0.0
     for q in qs:
         t1 = time.perf_counter()
-        plain_r.append(q.match(cap=cap, time_limit_ms=limit_ms, reorder=True)[0])
-        plain_rs.append(time.perf_counter() - t1)
-    keep_r = [i for i in range(nqueries) if plain_r[i].status == 0]
-    kq_r = [qs[i] for i in keep_r]
-    csm.batch_match(g, kq_r, opts=tuned, reorder=True)
+        plain_r.append(q.match(cap=cap, item_budget=item_budget, reorder=True)[0])
+        t_plain_r += time.perf_counter() - t1
+    csm.batch_match(g, qs, opts=tuned, reorder=True)
     t2 = time.perf_counter()
-    res_r = csm.batch_match(g, kq_r, opts=tuned, reorder=True)
+    res_r = csm.batch_match(g, qs, opts=tuned, reorder=True)
     t_batch_r = time.perf_counter() - t2
-    t_plain_r = sum(plain_rs[i] for i in keep_r)
-    print(f"# reordered: plain {len(keep_r)} queries in {t_plain_r:.2f}s, batched in {t_batch_r:.2f}s",
+    print(f"# reordered: plain {nqueries} queries in {t_plain_r:.2f}s, batched in {t_batch_r:.2f}s",
           file=sys.stderr, flush=True)
-    same_r = all((x.count, x.capped, x.status) == (plain_r[i].count, plain_r[i].capped, 0) for x, i in zip(res_r, keep_r))
-    same_default = all((plain_r[i].count, plain_r[i].capped) == (plain[i].count, plain[i].capped) for i in keep)
+    same_r = all((x.count, x.capped, x.status) == (p.count, p.capped, 0) for x, p in zip(res_r, plain_r))
+    same_default = all((plain_r[i].count, plain_r[i].capped) == (plain[i].count, plain[i].capped) for i in both)
+    # the queries the default order stopped: the reference-order batch's
+    # result against the reordered plain path's exact one (counts and cap
+    # flags do not depend on the order)
+    stopped = [i for i in range(nqueries) if plain[i].status != 0 or res[i].status != 0]
+    out["stopped_checked_by_reordered_plain"] = {
+        "queries": stopped,
+        "batch_equals_reordered_plain": all(
+            (res[i].count, res[i].capped, res[i].status) == (plain_r[i].count, plain_r[i].capped, 0)
+            for i in stopped if res[i].status == 0 and plain_r[i].status == 0),
+        "results": [[i, res[i].count, int(res[i].capped), res[i].status, plain_r[i].count, int(plain_r[i].capped),
+                     plain_r[i].status] for i in stopped],
+    }
     out["reordered"] = {
         "order": "candidate-aware (reorder option) on both paths; the default is the reference's BFS edge order",
-        "queries": len(keep_r),
-        "queries_per_s": round(len(keep_r) / t_batch_r, 2),
-        "plain_queries_per_s": round(len(keep_r) / t_plain_r, 2),
+        "queries": nqueries,
+        "queries_per_s": round(nqueries / t_batch_r, 2),
+        "plain_queries_per_s": round(nqueries / t_plain_r, 2),
         "speedup_vs_plain": round(t_plain_r / t_batch_r, 2),
-        "speedup_vs_default_plain": round((len(keep_r) / t_batch_r) / (len(keep) / t_plain), 2),
+        "speedup_vs_default_plain": round(t_plain / t_batch_r, 2),
         "counts_equal_plain": same_r,
         "counts_equal_default_order": same_default,
         "items_batch": sum(x.items for x in res_r),
@@ -477,15 +623,18 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cap", type=int, default=4_000_000_000)
+    ap.add_argument("--group", type=int, default=16, help="headline queries per step (one batch call)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c2-limits-ms", type=str, default="5000,45000",
+                    help="time limits of the two uncapped R-MAT 16 queries (tree, cycles+3)")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--batched-queries", type=int, default=256)
     ap.add_argument("--batched-cap", type=int, default=100_000_000)
-    ap.add_argument("--batched-limit-ms", type=int, default=2000)
-    ap.add_argument("--batched-item-budget", type=int, default=0)
+    ap.add_argument("--batched-item-budget", type=int, default=20_000_000_000)
+    ap.add_argument("--batched-limit-ms", type=int, default=2000, help="served workload: plain-path screening limit")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-sample", type=int, default=0,
                     help="profiling: batch only over the first N queries (no plain leg, no autotuner)")
@@ -495,91 +644,65 @@ def main():
     ap.add_argument("--served-more-qps", type=str, default="64,96",
                     help="comma-separated further arrival rates for the served workload ('' for none)")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="profiling: the headline steps only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; multi-GPU sharding is a design only "
                                    "(queries or root candidates split across GPUs, one u64 count reduction)"}))
         sys.exit(2)
 
-    t0 = time.time()
-    hg, named = build_workload()
     ctx = csm.Context(0, int(args.arena_gb * (1 << 30)))
-    g = ctx.upload(hg)
-    nv, ne, maxdeg = g.info()
-    queries = [g.prepare(hq.plan()) for _, hq in named]
-    print(f"# setup {time.time() - t0:.1f}s: graph |V|={nv} |E|={ne} max_degree={maxdeg}", file=sys.stderr, flush=True)
     ms_copy = ctx.copy_probe(1 << 30, 10)
     copy_gbs = 2.0 * (1 << 30) / (ms_copy * 1e6)
-
-    for _ in range(args.warmup):
-        run_step(queries, args.cap)
-    # timed steps; kernel device times from HIP events on the launch stream
-    ctx.set_profiling(True)
-    t1 = time.perf_counter()
-    steps = [run_step(queries, args.cap) for _ in range(args.steps)]
-    elapsed = time.perf_counter() - t1
-    ctx.set_profiling(False)
-    res = steps[-1]
-    ms_per_step = elapsed * 1e3 / args.steps
-    embeddings = sum(r.count for r in res)
-    value = embeddings / (elapsed / args.steps)
-    print(f"# timed {args.steps} steps in {elapsed:.2f}s: {embeddings} embeddings/step", file=sys.stderr, flush=True)
-    tot = kernel_totals(steps)
-
+    c4 = c4_section(ctx, args)
+    elapsed, tot = c4["elapsed"], c4["tot"]
+    nv, ne, maxdeg = c4["graph"]
     line = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": round(c4["embeddings"] / elapsed, 1),
         "unit": "embeddings/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": "plain path: R-MAT scale 16 (2^20 samples, 16 Zipf(1.0) labels), two 8-vertex "
-                        "random-walk queries (tree-like; +3 cycle edges), count-only",
+            "workload": f"batched path, Chung-Lu power-law graph (16M vertices, 256M edge samples, exponent 2.1, "
+                        f"max degree 50,000, 64 Zipf(0.8) labels), {C4_QUERIES} random-walk queries of 12..24 "
+                        f"vertices with 2..6 non-tree edges, count-only, cap 1e9 per query; one step = one "
+                        f"csm_batch_match over {args.group} of the queries (groups in turn)",
             "graph_vertices": nv,
             "graph_edges": ne,
-            "cap_per_query": args.cap,
+            "max_degree": maxdeg,
+            "queries": C4_QUERIES,
+            "queries_per_step": args.group,
+            "cap_per_query": C4_CAP,
             "parallelism": "single GPU",
-            "queries": [{"name": n, "count": r.count, "capped": bool(r.capped),
-                         "ms": round(r.ms_total, 3)} for (n, _), r in zip(named, res)],
+            **c4["detail"],
         },
-        "roofline": roofline_from(tot),
+        "roofline": roofline_from(tot, "batch_join_chung_lu_16m.md", cls=csm.K_BATCH),
         "kernels": kernels_json(tot),
         "copy_probe_GBps": round(copy_gbs, 1),
     }
-    for q in queries:
-        q.free()
-    # the same steps with the candidate-aware matching order (same counts)
-    rq = [g.prepare(hq.plan()) for _, hq in named]
-    for _ in range(args.warmup):
-        run_step(rq, args.cap, reorder=True)
-    t1 = time.perf_counter()
-    rsteps = [run_step(rq, args.cap, reorder=True) for _ in range(args.steps)]
-    r_elapsed = time.perf_counter() - t1
-    line["reordered"] = {
-        "order": "candidate-aware (reorder option); value above uses the reference's BFS edge order",
-        "value": round(sum(r.count for r in rsteps[-1]) / (r_elapsed / args.steps), 1),
-        "ms_per_step": round(r_elapsed * 1e3 / args.steps, 3),
-        "counts_equal_default_order": [(r.count, r.capped) for r in rsteps[-1]] == [(r.count, r.capped) for r in res],
-    }
-    for q in rq:
-        q.free()
-    g.free()
-    if not args.no_batched:
-        line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_limit_ms,
-                                          args.batched_item_budget, args.batched_sample)
-    if not args.no_served:
+    hg, hqs = c4["hg"], c4["hqs"]
+    del c4
+    if not args.headline_only and not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(hg, hqs[:4], C4_CAP, args.cpu_seconds)
+    del hg, hqs
+    if not args.headline_only and not args.no_c2:
+        lim = [int(x) for x in args.c2_limits_ms.split(",")]
+        line["c2_full"] = c2_section(ctx, lim)
+    if not args.headline_only and not args.no_batched:
+        line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_item_budget,
+                                          args.batched_sample)
+    if not args.headline_only and not args.no_served:
         more = [float(x) for x in args.served_more_qps.split(",") if x.strip()]
         line["served"] = served_section(ctx, args.served_queries, args.served_qps, args.batched_cap,
                                         args.batched_limit_ms, 64, 64, extra_qps=more)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(hg, [hq for _, hq in named], args.cap, args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -24,8 +24,9 @@ STATUS = {
     4: "query outside the supported envelope", 5: "planner limits exceeded", 6: "HIP runtime error",
     7: "device arena exhausted", 8: "no gfx950 device", 9: "serving queue full", 10: "unknown ticket",
     11: "result pending", 12: "graph exceeds the uint32 CSR layout", 13: "time limit reached",
+    14: "item budget reached",
 }
-E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING, E_TIMEOUT = 2, 3, 4, 7, 8, 9, 11, 13
+E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING, E_TIMEOUT, E_BUDGET = 2, 3, 4, 7, 8, 9, 11, 13, 14
 
 K_FILTER, K_REFINE, K_EXPAND, K_MATERIALIZE, K_COUNT, K_AUX, K_BATCH = range(7)
 K_NUM = 7
@@ -93,13 +94,14 @@ class QueryPlan(C.Structure):
 
 
 class KernelStat(C.Structure):
-    _fields_ = [("launches", C.c_uint64), ("ns", C.c_uint64), ("bytes", C.c_uint64)]
+    _fields_ = [("launches", C.c_uint64), ("ns", C.c_uint64), ("bytes", C.c_uint64), ("bytes_ref", C.c_uint64)]
 
 
 class MatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("reuse_filter", C.c_uint32),
                 ("list", C.POINTER(C.c_uint32)), ("list_capacity", C.c_uint64),
-                ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reorder", C.c_uint32)]
+                ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reorder", C.c_uint32),
+                ("item_budget", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -111,7 +113,7 @@ class Result(C.Structure):
 
     def kernel(self, cls):
         k = self.kernels[cls]
-        return {"launches": k.launches, "ns": k.ns, "bytes": k.bytes}
+        return {"launches": k.launches, "ns": k.ns, "bytes": k.bytes, "bytes_ref": k.bytes_ref}
 
 
 class BatchOpts(C.Structure):
@@ -453,13 +455,15 @@ class Query:
                              "csm_query_reorder")
 
     def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0,
-              time_limit_ms=0, reorder=False):
+              time_limit_ms=0, reorder=False, item_budget=0):
         """Returns (Result, rows or None); rows are sorted, in query-id order.
-        With time_limit_ms, Result.status == E_TIMEOUT marks a stopped search.
+        With time_limit_ms (item_budget), Result.status == E_TIMEOUT (E_BUDGET)
+        marks a stopped search whose count is a lower bound.
         reorder: candidate-aware matching order after the filter (same counts)."""
         o = MatchOpts()
         o.cap, o.refine, o.reuse_filter, o.chunk_items = cap, 1 if refine else 0, 1 if reuse_filter else 0, chunk_items
         o.time_limit_ms = time_limit_ms
+        o.item_budget = item_budget
         o.reorder = 1 if reorder else 0
         rows = None
         if want_list:
@@ -488,10 +492,10 @@ class Query:
         return r
 
 
-def batch_autotune(graph, queries, cap=0, refine=False, sample=32, time_limit_ms=0):
+def batch_autotune(graph, queries, cap=0, refine=False, sample=32, time_limit_ms=0, item_budget=0):
     """Tuned BatchOpts for this graph and query mix (csm_batch_autotune)."""
     arr = (_vp * len(queries))(*[q.h for q in queries])
-    base = BatchOpts(cap, 1 if refine else 0, 1, 0, 0, 0, 0, 0, 0, time_limit_ms, 0)
+    base = BatchOpts(cap, 1 if refine else 0, 1, 0, 0, 0, 0, 0, 0, time_limit_ms, item_budget)
     out = BatchOpts()
     graph.ctx.check(lib().csm_batch_autotune(graph.ctx.h, graph.h, arr, len(queries), C.byref(base), sample,
                                              C.byref(out)), "csm_batch_autotune")

@@ -112,6 +112,8 @@ struct BCtl {
   unsigned long long items;        // (row, list entry) pairs examined
   unsigned long long rows_in;      // rows read
   unsigned long long rows_out;     // rows written
+  unsigned long long bytes_ref;    // the reference algorithm's bytes for the rows processed (see finish())
+  unsigned long long bytes_own;    // this kernel's own logical bytes
   unsigned long long tasks;
   unsigned long long steals;
   unsigned long long fails;        // failed block allocations
@@ -154,7 +156,7 @@ struct BArgs {
   uint64_t block_words;            // u32 words per block
   uint32_t ilp;                    // items per thread per tile (1..kIlp)
   uint32_t max_out;                // full output blocks a task holds before it yields
-  uint64_t budget;                 // items per query before it is handed to the plain path
+  uint64_t budget;                 // items per query before it stops with a lower bound (the caller's budget)
   uint32_t cache_cap;              // free blocks a workgroup may keep (<= kCache)
   uint64_t deadline_ticks;         // s_memrealtime ticks (100 MHz) before the kernel gives up
 };
@@ -277,7 +279,8 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     st_init(a.free_head, 0ull);
     st_init(a.free_tail, static_cast<unsigned long long>(a.num_blocks));
     BCtl &c = *a.ctl;
-    unsigned long long *c64[] = {&c.root_next, &c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out, &c.tasks,
+    unsigned long long *c64[] = {&c.root_next, &c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out,
+                                 &c.bytes_ref, &c.bytes_own, &c.tasks,
                                  &c.steals, &c.fails, &c.busy_ticks, &c.idle_ticks, &c.sync_ticks, &c.phase_ticks[0],
                                  &c.phase_ticks[1], &c.phase_ticks[2], &c.phase_ticks[3], &c.alloc_fail,
                                  &c.late_recs, &c.bad_recs};
@@ -484,9 +487,10 @@ __device__ inline uint32_t row_of(const Smem &sm, uint32_t nb, uint64_t it) {
 
 // Membership of v[j] in the sorted lists cn[lo[j] .. lo[j] + n[j]), all
 // searches advancing together so their loads overlap.  n[j] = 0: not found.
+// `loads` gains the number of list entries read.
 template <uint32_t N>
 __device__ inline void lockstep_find(const uint32_t *cn, uint64_t (&lo)[N], uint32_t (&n)[N], const uint32_t (&v)[N],
-                                     bool (&found)[N]) {
+                                     bool (&found)[N], unsigned long long &loads) {
   bool has[N];
 #pragma unroll
   for (uint32_t j = 0; j < N; ++j) has[j] = n[j] != 0;
@@ -501,19 +505,23 @@ __device__ inline void lockstep_find(const uint32_t *cn, uint64_t (&lo)[N], uint
         const uint32_t half = n[j] >> 1;
         if (cn[lo[j] + half] <= v[j]) lo[j] += half;
         n[j] -= half;
+        ++loads;
       }
     }
   }
 #pragma unroll
-  for (uint32_t j = 0; j < N; ++j) found[j] = has[j] && cn[lo[j]] == v[j];
+  for (uint32_t j = 0; j < N; ++j) {
+    found[j] = has[j] && cn[lo[j]] == v[j];
+    loads += has[j] ? 1 : 0;
+  }
 }
 
-__device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, uint32_t v) {
+__device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, uint32_t v, unsigned long long &loads) {
   uint64_t lo[1] = {beg};
   uint32_t n[1] = {len};
   const uint32_t vv[1] = {v};
   bool f[1];
-  lockstep_find<1>(cn, lo, n, vv, f);
+  lockstep_find<1>(cn, lo, n, vv, f, loads);
   return f[0];
 }
 
@@ -576,7 +584,8 @@ __device__ uint32_t read_rec(const BArgs &a, unsigned long long ent, Task &t, bo
 
 // Process one task; returns with all threads converged.
 __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_t wg, unsigned long long &n_items,
-                             unsigned long long &n_rows_in, unsigned long long &n_rows_out) {
+                             unsigned long long &n_rows_in, unsigned long long &n_rows_out,
+                             unsigned long long &b_ref, unsigned long long &b_own) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const Task t = sm.task;
   const BQuery &Q = sm.q;
@@ -623,6 +632,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     grows = a.pool + static_cast<uint64_t>(t.block) * a.block_words + static_cast<uint64_t>(t.r0) * k;
   }
   unsigned long long contrib = 0, items_seen = 0, rows_written = 0;
+  unsigned long long probe_loads = 0;  // candidate-list entries read by membership searches
   uint32_t row0 = t.skip_rows;
   uint32_t skip_items = t.skip_items;
   uint32_t fill = 0;  // rows in the current output block (same value in every thread)
@@ -632,6 +642,10 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     sm.nheld = 0;
     sm.out_block = kNil;
     sm.tile_n[0] = sm.tile_n[1] = sm.tile_n[2] = 0;
+    if (sm.acc_q != t.q) {  // counts held back belong to one query at a time
+      flush_acc(a, sm);
+      sm.acc_q = t.q;
+    }
   }
   __syncthreads();
   uint64_t tp0 = __builtin_amdgcn_s_memrealtime();
@@ -687,6 +701,14 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       sm.dbeg[tid] = bbeg;
       sm.drv[tid] = static_cast<uint8_t>(bi);
       len = best;
+      {  // byte accounting: the reference's extension of this row reads the
+         // row and its parent's offsets and scans the parent's adjacency (id
+         // + candidate word per entry); this kernel reads the row and one
+         // rank word, prefix, list start and length per list it looks up
+        const uint32_t xp = row[sm.lv.parent];
+        b_ref += 4ull * k + 8 + 12ull * (a.probe.off[xp + 1] - a.probe.off[xp]);
+        b_own += 4ull * k + 28ull * (nanchor + (fused_rows ? 1 : 0));
+      }
       if (fused_rows) {  // the row's part of the last position's count
         const uint32_t xv = row[sm.lv_last.parent];
         uint64_t lb, le;
@@ -696,7 +718,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
           const uint32_t col = sm.lv_last.inj[x];
           if (col == k) continue;
           const uint32_t v = row[col];
-          if (v != xv && in_list(Q.ci.cn_adj, lb, static_cast<uint32_t>(le - lb), v)) --c;
+          if (v != xv && in_list(Q.ci.cn_adj, lb, static_cast<uint32_t>(le - lb), v, probe_loads)) --c;
         }
         sm.fbase[tid] = c;
         sm.fbeg[tid] = lb;
@@ -705,7 +727,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       if (row_counted) {  // one anchor: the row's survivors are its list minus the row's own ids
         uint64_t surv = len;
         for (uint32_t x = 0; x < sm.lv.ninj; ++x)
-          if (in_list(Q.ci.cn_adj, bbeg, static_cast<uint32_t>(len), row[sm.lv.inj[x]])) --surv;
+          if (in_list(Q.ci.cn_adj, bbeg, static_cast<uint32_t>(len), row[sm.lv.inj[x]], probe_loads)) --surv;
         contrib += surv * (fused_rows ? sm.fbase[tid] : 1u);
         items_seen += len;
         len = 0;
@@ -761,7 +783,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
             lo[j] = sm.abeg[x][ri[j]];
             n[j] = act ? sm.alen[x][ri[j]] : 0u;
           }
-          lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f);
+          lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f, probe_loads);
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j)
             if (ok[j] && sm.drv[ri[j]] != x) ok[j] = f[j];
@@ -786,7 +808,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
             n[j] = act ? sm.flen[ri[j]] : 0u;
           }
           if (last_inj_k) {
-            lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f);
+            lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f, probe_loads);
 #pragma unroll
             for (uint32_t j = 0; j < kIlp; ++j) contrib -= f[j] ? 1u : 0u;
           }
@@ -810,7 +832,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
               lo[j] = lb[j];
               n[j] = ok[j] && v[j] != w[j] ? static_cast<uint32_t>(le[j] - lb[j]) : 0u;
             }
-            lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, v, f);
+            lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, v, f, probe_loads);
 #pragma unroll
             for (uint32_t j = 0; j < kIlp; ++j) contrib -= f[j] ? 1u : 0u;
           }
@@ -889,6 +911,22 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       rows_written += tid == 0 ? tile : 0;
       fill += tile;
     }
+    if (count_level) {
+      // the row batch's count is added at once, so the query's total passes
+      // its cap within one row batch of the embedding that crosses it and
+      // every workgroup stops its tasks at their next row batch
+      const unsigned long long part = block_reduce(contrib, sm.red);
+      const unsigned long long its = block_reduce(items_seen, sm.red);
+      b_own += 4ull * items_seen;
+      if (tid == 0) {
+        sm.acc_count += part;
+        sm.acc_items += its;
+        n_items += its;
+        if (sm.acc_count >= Q.cap / (4ull * gridDim.x) + 1 || sm.acc_items >= (1ull << 22)) flush_acc(a, sm);
+      }
+      contrib = 0;
+      items_seen = 0;
+    }
     if (tid == 0) {
       const uint64_t tn = __builtin_amdgcn_s_memrealtime();
       sm.phase[count_level ? 1 : 2] += tn - tp0;
@@ -903,6 +941,9 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   __threadfence();
   const unsigned long long tot_contrib = block_reduce(contrib, sm.red);
   const unsigned long long tot_items = block_reduce(items_seen, sm.red);
+  b_own += 4ull * items_seen + 4ull * probe_loads;   // one list entry per item, the searches' entries
+  b_ref += 4ull * kout * rows_written;               // rows written (tid 0 holds the tile counts)
+  b_own += 4ull * kout * rows_written;
   if (tid == 0) {
     const bool live = ld_done(a, t.q) == kRunning && !aborted(a);
     if (!live && ld_done(a, t.q) == kRunning) a.st[t.q].lost = 1;  // its outputs are dropped by an early stop
@@ -966,6 +1007,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
   const uint64_t nroot = static_cast<uint64_t>(a.nqueries) * a.tasks_per_query;
   uint64_t busy = 0;
   unsigned long long n_items = 0, n_rows_in = 0, n_rows_out = 0, n_tasks = 0, n_steals = 0;
+  unsigned long long b_ref = 0, b_own = 0;  // per thread, summed at exit
   uint32_t rng = 0x9E3779B9u ^ (wg * 0x85EBCA6Bu + 1u);
   bool idle = false;  // counted in ctl->idle
   uint32_t idle_rounds = 0;
@@ -1108,7 +1150,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       sm.staged_k = k;
     }
     if (!sm.stop) {
-      process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out);
+      process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out, b_ref, b_own);
       if (sm.flag == 2)  // pool empty: give other workgroups time to free blocks
         for (int i = 0; i < 16; ++i) __builtin_amdgcn_s_sleep(32);
     } else if (tid == 0) {
@@ -1120,8 +1162,12 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
     ++n_tasks;
     busy += __builtin_amdgcn_s_memrealtime() - t_task;
   }
+  const unsigned long long wg_ref = block_reduce(b_ref, sm.red);
+  const unsigned long long wg_own = block_reduce(b_own, sm.red);
   if (tid == 0) {
     flush_acc(a, sm);
+    atomicAdd(&a.ctl->bytes_ref, wg_ref);
+    atomicAdd(&a.ctl->bytes_own, wg_own);
     const uint64_t idle_t = __builtin_amdgcn_s_memrealtime() - t_start - busy;
     atomicAdd(&a.ctl->busy_ticks, static_cast<unsigned long long>(busy));
     atomicAdd(&a.ctl->idle_ticks, static_cast<unsigned long long>(idle_t));
@@ -1215,6 +1261,8 @@ csm_status BatchRun::prepare(hipStream_t s) {
     for (uint32_t i = 0; i < nqs; ++i) {
       for (uint32_t u = 0; u < queries_[i]->desc.nq; ++u) queries_[i]->counts[u] = host[i * kStatSize + u];
       queries_[i]->filtered = true;
+      add_filter_bytes(g, queries_[i], host.data() + static_cast<size_t>(i) * kStatSize, o_.refine != 0,
+                       kstat_.data());
     }
   }
   if (o_.reorder)  // candidate-aware order from these counts: host work only
@@ -1283,9 +1331,9 @@ csm_status BatchRun::launch(hipStream_t s) {
   const uint32_t nqs = static_cast<uint32_t>(queries_.size()), words = words_;
   TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, o.block_kb ? o.block_kb : 128u,
                 o.ilp ? std::min(o.ilp, kIlp) : kIlp, o.max_out ? std::min(o.max_out, kMaxOut) : 1u};
-  // with a time limit the queries are bounded by time; otherwise by items, so
-  // an unlucky depth-first search is handed to the plain path and stays exact
-  const uint64_t budget = o.item_budget ? o.item_budget : o.time_limit_ms ? ~0ull : 5000000000ull;
+  // the caller's item budget stops a query with a lower bound; without one a
+  // query runs in the kernel until it is done (or the kernel's time limit)
+  const uint64_t budget = o.item_budget ? o.item_budget : ~0ull;
   if (o.chunk_rows) tp.root_words = std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, o.chunk_rows / 64));
   if (tp.wgs_per_cu > 8 || tp.block_kb < 32 || tp.block_kb > 4096) return set_error(ctx, CSM_E_ARG, "batch tuning out of range");
   const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;
@@ -1380,9 +1428,16 @@ csm_status BatchRun::finish(csm_result *results) {
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
   join_kt_->flush();
   release_graph();
-  // algorithmic bytes of the persistent join: rows read, per-row anchor list
-  // lookups, list entries, rows written
-  kstat_[CSM_K_BATCH].bytes = hctl.rows_in * (4ull * 8 + 32) + hctl.items * 4 + hctl.rows_out * 4ull * 9;
+  // algorithmic bytes of the persistent join: its logical reads and writes,
+  // per input row of level k the row (4k) and a rank word, prefix, list start
+  // and length per candidate list looked up (28), 4 per list entry examined
+  // and per entry read by a membership search, 4(k + 1) per row written.
+  // bytes_ref: the reference's extension pass (join.cu:99-177) over the same
+  // rows, 4k + 8 (offsets) + 12 per adjacency entry of the row's parent vertex
+  // (id + candidate word) + 4(k + 1) per row written, non-tree checks and the
+  // counted last position not charged: the work the candidate index avoids.
+  kstat_[CSM_K_BATCH].bytes = hctl.bytes_own;
+  kstat_[CSM_K_BATCH].bytes_ref = hctl.bytes_ref;
   const char *tr = std::getenv("CSM_TRACE");
   const bool trace = tr && *tr == '1';
   const uint32_t grid = grid_;
@@ -1431,10 +1486,11 @@ csm_status BatchRun::finish(csm_result *results) {
       r.kernels[CSM_K_BATCH].launches = 1;
     }
   }
-  // Queries over their item budget, and every unfinished query after the
-  // kernel stopped early, are finished by the plain path (one at a time,
-  // exact), except that with a caller's time limit a query still unfinished
-  // when the limit passed keeps its count so far with status CSM_E_TIMEOUT.
+  // Every unfinished query after the kernel stopped early is finished by the
+  // plain path (one at a time, exact), except that with a caller's time limit
+  // a query still unfinished when the limit passed keeps its count so far
+  // with status CSM_E_TIMEOUT, and with a caller's item budget a query over it
+  // keeps its count so far with status CSM_E_BUDGET.
   // A running query whose task accounting did not close (tasks pending, or
   // one dropped) is finished by the plain path too: its count is not trusted.
   uint32_t fallbacks = 0, unclosed = 0;
@@ -1445,6 +1501,10 @@ csm_status BatchRun::finish(csm_result *results) {
     if (hst[i].done == kRunning && finished) continue;  // complete before any early stop: exact
     if (o_.time_limit_ms && hctl.abort == kAbortTime && hst[i].done == kRunning) {
       results[i].status = CSM_E_TIMEOUT;
+      continue;
+    }
+    if (o_.item_budget && hst[i].done == kAbandoned) {  // the caller's bound: a lower bound, as in csm_match
+      results[i].status = CSM_E_BUDGET;
       continue;
     }
     {
@@ -1469,7 +1529,7 @@ csm_status BatchRun::finish(csm_result *results) {
       results[i].status = r.status;
       results[i].items += r.items;
       results[i].partial_rows += r.partial_rows;
-      results[i].path = hst[i].done == kAbandoned ? CSM_PATH_BATCH_BUDGET : CSM_PATH_BATCH_ABORT;
+      results[i].path = CSM_PATH_BATCH_ABORT;
       ++fallbacks;
     }
   }

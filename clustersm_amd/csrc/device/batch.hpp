@@ -20,9 +20,9 @@ class BatchRun {
   csm_status prepare(hipStream_t s);
   // Pool, deques and the persistent join kernel, queued on stream s.
   csm_status launch(hipStream_t s);
-  // Waits for the join, fills results[i] for queries[i]; queries over their
-  // budget (or all unfinished ones after an abort) are finished by the plain
-  // path on ctx->stream.
+  // Waits for the join, fills results[i] for queries[i]; after an early stop
+  // of the kernel its unfinished queries are finished by the plain path on
+  // ctx->stream.
   csm_status finish(csm_result *results);
 
   const std::vector<csm_query *> &queries() const { return queries_; }

@@ -249,6 +249,7 @@ class KernelTimer {
   }
   void end(int cls, uint64_t bytes) {
     stats_[cls].bytes += bytes;
+    stats_[cls].bytes_ref += bytes;
     if (!ctx_->profiling) return;
     hipEvent_t e = get();
     hipEventRecord(e, ctx_->stream);

@@ -345,6 +345,25 @@ FilterQ filter_entry(const csm_graph *g, csm_query *q, uint64_t *stats) {
   return f;
 }
 
+// Algorithmic bytes of one query's filter steps, from its statistics block:
+// labels + offsets of every vertex, (neighbour id + neighbour label) per
+// scanned adjacency entry, bitmap words written; refinement: (adjacency id +
+// bitmap word) per probe, bitmap words read; counting: bitmap words read.
+void add_filter_bytes(const csm_graph *g, const csm_query *q, const uint64_t *stats, bool refine, csm_kernel_stat *ks) {
+  const uint64_t words = q->desc.words, nq = q->desc.nq;
+  const uint64_t fb = 8ull * g->n + 4 + 8ull * stats[kStatFilterScanned] + 8ull * words * nq;
+  uint64_t clusters_bytes = 0;
+  for (uint32_t c = 0; c < q->plan.num_clusters; ++c)
+    clusters_bytes += 24ull * words * (q->plan.cluster_offsets[c + 1] - q->plan.cluster_offsets[c]);
+  const uint64_t rb = refine ? 12ull * stats[kStatRefineScanned] + clusters_bytes : 0;
+  ks[CSM_K_FILTER].bytes += fb;
+  ks[CSM_K_FILTER].bytes_ref += fb;
+  ks[CSM_K_REFINE].bytes += rb;
+  ks[CSM_K_REFINE].bytes_ref += rb;
+  ks[CSM_K_AUX].bytes += 8ull * words * nq;
+  ks[CSM_K_AUX].bytes_ref += 8ull * words * nq;
+}
+
 // Filter + refinement for one query on ctx->stream; fills q->counts.
 csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine, WorkMem *mem) {
   auto t0 = std::chrono::steady_clock::now();
@@ -367,15 +386,7 @@ csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     kt.flush();
     for (uint32_t u = 0; u < nq; ++u) q->counts[u] = host[u];
-    // algorithmic bytes: labels + offsets of every vertex, (neighbour id +
-    // neighbour label) per scanned adjacency entry, bitmap words written;
-    // refinement: (adjacency id + bitmap word) per probe, bitmap words read
-    q->kstat[CSM_K_FILTER].bytes = 8ull * g->n + 4 + 8ull * host[kStatFilterScanned] + 8ull * words * nq;
-    uint64_t clusters_bytes = 0;
-    for (uint32_t c = 0; c < q->plan.num_clusters; ++c)
-      clusters_bytes += 24ull * words * (q->plan.cluster_offsets[c + 1] - q->plan.cluster_offsets[c]);
-    q->kstat[CSM_K_REFINE].bytes = refine ? 12ull * host[kStatRefineScanned] + clusters_bytes : 0;
-    q->kstat[CSM_K_AUX].bytes += 8ull * words * nq;
+    add_filter_bytes(g, q, host, refine != 0, q->kstat);
   }
   q->filtered = true;
   q->ms_filter = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

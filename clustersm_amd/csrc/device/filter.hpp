@@ -30,6 +30,10 @@ std::vector<uint32_t> step_members_of(const std::vector<const csm_query *> &qs);
 void launch_filter_batch(hipStream_t s, const csm_graph *g, const FilterQ *d_qs, uint32_t nqueries, uint32_t words,
                          const std::vector<uint32_t> &step_members, uint32_t max_nq, bool refine, KernelTimer *kt);
 
+// Adds one query's filter, refinement and counting bytes to ks, from the
+// query's statistics block (host copy).
+void add_filter_bytes(const csm_graph *g, const csm_query *q, const uint64_t *stats, bool refine, csm_kernel_stat *ks);
+
 // Filter + refinement for one query on ctx->stream; its scratch comes from
 // `mem` (nullptr: the context arena).
 csm_status run_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine, WorkMem *mem = nullptr);

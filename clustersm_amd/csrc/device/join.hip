@@ -240,14 +240,30 @@ __global__ __launch_bounds__(256) void reduce_add_kernel(const uint64_t *vals, u
 uint32_t reduce_grid(uint64_t n) { return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(256, n / 2048))); }
 
 // tile_row[t] = row (relative to a) holding item t * kTile of the chunk
-// [a, b); tile_row[ntiles] = b - a.
-__global__ void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, uint64_t ntiles, uint32_t *tile_row) {
+// [a, b); tile_row[ntiles] = b - a.  For the byte accounting, *deg_acc gains
+// the degree of each row's vertex at column `parent` (one atomic per block).
+__global__ __launch_bounds__(256) void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, uint64_t ntiles,
+                                                        uint32_t *tile_row, const uint32_t *rows, uint32_t k,
+                                                        uint32_t parent, const uint32_t *off,
+                                                        unsigned long long *deg_acc) {
+  __shared__ unsigned long long lds[256 / kWave];
   const uint64_t base = dscan[a];
+  unsigned long long deg = 0;
   for (uint64_t r = a + blockIdx.x * 256ull + threadIdx.x; r < b; r += 256ull * gridDim.x) {
     const uint64_t lo = dscan[r] - base, hi = dscan[r + 1] - base;  // items [lo, hi)
     for (uint64_t t = (lo + kTile - 1) / kTile; t * kTile < hi; ++t) tile_row[t] = static_cast<uint32_t>(r - a);
+    const uint32_t x = rows[r * k + parent];
+    deg += off[x + 1] - off[x];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) tile_row[ntiles] = static_cast<uint32_t>(b - a);
+  for (int d = kWave / 2; d > 0; d >>= 1) deg += __shfl_xor(deg, d, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = deg;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
+    if (t) atomicAdd(deg_acc, t);
+  }
 }
 
 // Per-row pass of a level.
@@ -376,7 +392,9 @@ enum : int {
   kCtlTotal = 1,     // scan total (rows materialised by the last step)
   kCtlChunkA = 2,    // chunk choice for a new level (b, items, base)
   kCtlChunkB = 5,    // chunk choice for the current level (b, items, base)
-  kCtlSize = 8
+  kCtlDegMark = 8,   // parent degrees summed over the rows of marked chunks
+  kCtlDegCount = 9,  // and over the rows of counted chunks
+  kCtlSize = 10
 };
 
 // List mode.  Query vertex of each matching-order position, by value.
@@ -565,6 +583,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
 
   std::vector<uint32_t> host_rows;
   uint64_t total = 0;
+  uint64_t ref_mark = 0, ref_count = 0, ref_mat = 0;  // reference-algorithm bytes by class (host part)
   uint32_t k = 1;
   bool stop_all = false;
 
@@ -631,6 +650,10 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
       res.status = CSM_E_TIMEOUT;  // count so far is a lower bound
       break;
     }
+    if (o.item_budget && res.items >= o.item_budget) {
+      res.status = CSM_E_BUDGET;  // count so far is a lower bound
+      break;
+    }
     if (cursor[k] >= nrows[k]) {
       if (k == 1) break;
       --k;
@@ -665,9 +688,15 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     const uint32_t ntiles = ceil_div(items, kTile);
     const uint64_t rows_in = b - a;
     const size_t lds_rows = 4ull * kTileRows * k;
+    const bool counted = k == stop && !want_list;
     kt.begin(CSM_K_AUX);
-    tile_rows_kernel<<<grid_for(rows_in), 256, 0, s>>>(ws.dscan[k], a, b, ntiles, ws.tile_row);
+    tile_rows_kernel<<<grid_for(rows_in), 256, 0, s>>>(
+        ws.dscan[k], a, b, ntiles, ws.tile_row, ws.rows[k], k, lv[k].parent, g->off,
+        reinterpret_cast<unsigned long long *>(ws.ctl + (counted ? kCtlDegCount : kCtlDegMark)));
     kt.end(CSM_K_AUX, rows_in * 16ull + 4ull * ntiles);
+    // the reference's extension pass over these rows, without the scan of the
+    // parents' adjacency (added from the device sums at the end)
+    (counted ? ref_count : ref_mark) += rows_in * (4ull * k + 8);
     res.items += items;
     // algorithmic bytes of one step: rows read, driver offsets, (adjacency id
     // + candidate bitmap word) per item; the rows written are added below
@@ -711,6 +740,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
       kt.begin(CSM_K_MATERIALIZE);
       expand_kernel<kMaterialize><<<ntiles, kTile, lds_rows, s>>>(ea);
       kt.end(CSM_K_MATERIALIZE, rows_in * (4ull * k + 9) + items * (4ull + 1) + produced * 4ull * (k + 1));
+      ref_mat += produced * 4ull * (k + 1);
     }
     res.partial_rows += produced;
     cursor[k] = b;
@@ -743,6 +773,17 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     }
   }
   kt.flush();
+  {  // join classes: bytes_ref = the reference's extension of the same rows
+     // (join.cu:99-177): 4k + 8 per row, 12 per adjacency entry of its parent
+     // vertex (id + candidate word), 4(k + 1) per row written; the work the
+     // candidate index avoids (bytes: the kernels' own reads and writes,
+     // without the entries read by membership searches)
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    res.kernels[CSM_K_EXPAND].bytes_ref = ref_mark + 12ull * pin[kCtlDegMark];
+    res.kernels[CSM_K_COUNT].bytes_ref = ref_count + 12ull * pin[kCtlDegCount];
+    res.kernels[CSM_K_MATERIALIZE].bytes_ref = ref_mat;
+  }
   res.count = std::min(total, cap);
   res.capped = total > cap;
   if (want_list && dlist) {
@@ -905,7 +946,7 @@ extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, 
   for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
   out->ms_filter = q->ms_filter;
   csm_status st = run_join(ctx, g, q, o, *out);
-  if (st != CSM_OK || out->status != CSM_E_TIMEOUT) out->status = st;
+  if (st != CSM_OK || (out->status != CSM_E_TIMEOUT && out->status != CSM_E_BUDGET)) out->status = st;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
@@ -942,7 +983,7 @@ extern "C" csm_status csm_match_noalloc(csm_ctx *ctx, const csm_graph *g, csm_qu
   if (st == CSM_E_NOMEM)
     set_error(ctx, st, "workspace beyond the static scratch of " + std::to_string(kScratchBytes >> 20) +
                            " MiB (" + ctx->last_error + ")");
-  if (st != CSM_OK || out->status != CSM_E_TIMEOUT) out->status = st;
+  if (st != CSM_OK || (out->status != CSM_E_TIMEOUT && out->status != CSM_E_BUDGET)) out->status = st;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }

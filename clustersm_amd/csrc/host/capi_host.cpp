@@ -47,6 +47,7 @@ const char *csm_status_string(csm_status s) {
     case CSM_E_PENDING: return "result pending";
     case CSM_E_LIMIT: return "graph exceeds the uint32 CSR layout";
     case CSM_E_TIMEOUT: return "time limit reached";
+    case CSM_E_BUDGET: return "item budget reached";
   }
   return "unknown status";
 }

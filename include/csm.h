@@ -44,7 +44,8 @@ typedef enum csm_status {
   CSM_E_NOT_FOUND = 10, /* unknown serving ticket */
   CSM_E_PENDING = 11,   /* serving result not ready yet */
   CSM_E_LIMIT = 12,     /* data graph beyond the int32 CSR layout (2|E| >= 2^32) */
-  CSM_E_TIMEOUT = 13    /* result status only: the time limit ended the search; count is a lower bound */
+  CSM_E_TIMEOUT = 13,   /* result status only: the time limit ended the search; count is a lower bound */
+  CSM_E_BUDGET = 14     /* result status only: the caller's item budget ended the search; count is a lower bound */
 } csm_status;
 
 typedef struct csm_host_graph csm_host_graph; /* parsed or generated graph in host memory */
@@ -188,6 +189,9 @@ typedef struct csm_match_opts {
   uint32_t reorder;        /* 1: candidate-aware matching order after the filter, as csm_query_reorder
                               (same counts; a capped list holds the first embeddings of that order;
                               the query keeps the new order) */
+  uint64_t item_budget;    /* 0 = none; else the search stops once about this many items (row x candidate
+                              tests) are examined: the result's status is CSM_E_BUDGET with the count so
+                              far (the call itself returns CSM_OK) */
 } csm_match_opts;
 
 /* Kernel classes for per-kernel accounting. */
@@ -204,7 +208,10 @@ enum {
 typedef struct csm_kernel_stat {
   uint64_t launches;
   uint64_t ns;             /* summed device time (only with csm_set_profiling) */
-  uint64_t bytes;          /* algorithmic bytes */
+  uint64_t bytes;          /* algorithmic bytes: the logical reads and writes of the kernels' algorithm */
+  uint64_t bytes_ref;      /* join classes: the reference algorithm's bytes for the same rows (extension by
+                              a scan of each row's parent adjacency, join.cu:99-177), which the candidate
+                              index avoids; the other classes: equal to bytes */
 } csm_kernel_stat;
 
 typedef struct csm_result {
@@ -230,7 +237,7 @@ enum {
   CSM_PATH_PLAIN = 0,       /* csm_match */
   CSM_PATH_HOST = 1,        /* answered without a join (two vertices, or an empty candidate set) */
   CSM_PATH_BATCH = 2,       /* the persistent batch kernel */
-  CSM_PATH_BATCH_BUDGET = 3,/* over the batch item budget: finished by the plain path */
+  CSM_PATH_BATCH_BUDGET = 3,/* not produced any more (an item budget stops a query with CSM_E_BUDGET) */
   CSM_PATH_BATCH_ABORT = 4  /* the batch kernel gave up (time limit, pool): finished by the plain path */
 };
 
@@ -290,8 +297,8 @@ typedef struct csm_batch_opts {
   uint32_t time_limit_ms;  /* 0: none (the kernel still stops after 120 s and the plain path finishes
                               its queries, exact); else per query: a query unfinished after this has
                               status CSM_E_TIMEOUT and its count so far, as with csm_match */
-  uint64_t item_budget;    /* items per query before the plain path takes it over; 0 = 5e9 without a
-                              time limit, unlimited with one */
+  uint64_t item_budget;    /* 0 = none; else, as in csm_match_opts, a query over about this many items
+                              stops with status CSM_E_BUDGET and its count so far */
   uint32_t reorder;        /* 1: candidate-aware matching order for every query, from the batch's own
                               filter counts (as csm_query_reorder; same counts; the queries keep it) */
   uint32_t reserved;

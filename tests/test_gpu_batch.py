@@ -108,12 +108,23 @@ def test_batch_tuning_variants_agree(ctx):
     cap = 500_000
     want = [(x.count, x.capped) for x in (q.match(cap=cap)[0] for q in qs)]
     for kw in (dict(wgs_per_cu=1), dict(wgs_per_cu=3, max_out=1), dict(block_kb=32, ilp=1), dict(max_out=8, ilp=2),
-               dict(chunk_rows=64), dict(item_budget=1000)):
+               dict(chunk_rows=64)):
         print("tuning", kw, flush=True)
         res = csm.batch_match(g, qs, cap=cap, time_limit_ms=20000, **kw)
         assert [(x.count, x.capped) for x in res] == want, kw
-        if "item_budget" not in kw:
-            assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) and x.anomalies == 0 for x in res), kw
+        assert all(x.path in (csm.PATH_BATCH, csm.PATH_HOST) and x.anomalies == 0 for x in res), kw
+    # an item budget stops a query with a lower bound; queries within it are exact
+    res = csm.batch_match(g, qs, cap=cap, time_limit_ms=20000, item_budget=1000)
+    assert any(x.status == csm.E_BUDGET for x in res)
+    for x, w in zip(res, want):
+        if x.status == 0:
+            assert (x.count, x.capped) == w
+        else:
+            assert x.status == csm.E_BUDGET and x.count <= w[0] and not x.capped
+    plain = [q.match(cap=cap, item_budget=1000)[0] for q in qs]
+    assert any(p.status == csm.E_BUDGET for p in plain)
+    for p, w in zip(plain, want):
+        assert (p.count, p.capped) == w if p.status == 0 else (p.status == csm.E_BUDGET and p.count <= w[0])
     for q in qs:
         q.free()
     g.free()
