@@ -292,6 +292,7 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
 }
 
 constexpr uint32_t kIlp = 4;     // items per thread per tile, checked in lockstep
+constexpr uint64_t kHubTag = 1ull << 63;  // abeg entry of a hub anchor: tag | hub index
 constexpr uint32_t kMaxA = 4;    // anchors whose candidate lists are kept per row
 constexpr uint32_t kMaxOut = 8;  // most full output blocks a task holds before it yields
 constexpr uint32_t kCache = 16;  // free blocks a workgroup keeps for itself
@@ -633,6 +634,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   }
   unsigned long long contrib = 0, items_seen = 0, rows_written = 0;
   unsigned long long probe_loads = 0;  // candidate-list entries read by membership searches
+  unsigned long long hub_probes = 0;   // adjacency-bitmap words read
   uint32_t row0 = t.skip_rows;
   uint32_t skip_items = t.skip_items;
   uint32_t fill = 0;  // rows in the current output block (same value in every thread)
@@ -701,6 +703,13 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       sm.dbeg[tid] = bbeg;
       sm.drv[tid] = static_cast<uint8_t>(bi);
       len = best;
+      // kept anchors other than the driver whose vertex is a hub are tested
+      // against its adjacency bitmap (the driver's start is taken above)
+      for (uint32_t x = 0; x < nkept; ++x) {
+        if (x == bi) continue;
+        const int32_t h = a.probe.hub_id[row[sm.lv.anchor[x]]];
+        if (h >= 0) sm.abeg[x][tid] = kHubTag | static_cast<uint64_t>(h);
+      }
       {  // byte accounting: the reference's extension of this row reads the
          // row and its parent's offsets and scans the parent's adjacency (id
          // + candidate word per entry); this kernel reads the row and one
@@ -774,19 +783,29 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       // in the anchor's candidate list; binary searches run in lockstep
       for (uint32_t x = 0; x < nanchor; ++x) {
         if (x < kMaxA) {
+          // an anchor with a dense adjacency bitmap (abeg holds kHubTag | its
+          // hub index): one word tells w ~ r[x], since w is a candidate of
+          // u_k already; the others search the anchor's candidate list
           uint64_t lo[kIlp];
           uint32_t n[kIlp];
-          bool f[kIlp];
+          bool f[kIlp], hub[kIlp], hbit[kIlp];
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j) {
             const bool act = ok[j] && sm.drv[ri[j]] != x;
-            lo[j] = sm.abeg[x][ri[j]];
-            n[j] = act ? sm.alen[x][ri[j]] : 0u;
+            const uint64_t b = sm.abeg[x][ri[j]];
+            const bool tagged = (b & kHubTag) != 0;
+            hub[j] = act && tagged;
+            lo[j] = tagged ? 0 : b;  // a tagged entry is no list position
+            hbit[j] = false;
+            if (hub[j])
+              hbit[j] = (a.probe.hub_bits[(b & ~kHubTag) * a.words + (w[j] >> 6)] >> (w[j] & 63)) & 1ull;
+            n[j] = act && !tagged ? sm.alen[x][ri[j]] : 0u;
+            hub_probes += hub[j] ? 1 : 0;
           }
           lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, w, f, probe_loads);
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j)
-            if (ok[j] && sm.drv[ri[j]] != x) ok[j] = f[j];
+            if (ok[j] && sm.drv[ri[j]] != x) ok[j] = hub[j] ? hbit[j] : f[j];
         } else {
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j)
@@ -941,7 +960,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   __threadfence();
   const unsigned long long tot_contrib = block_reduce(contrib, sm.red);
   const unsigned long long tot_items = block_reduce(items_seen, sm.red);
-  b_own += 4ull * items_seen + 4ull * probe_loads;   // one list entry per item, the searches' entries
+  b_own += 4ull * items_seen + 4ull * probe_loads + 8ull * hub_probes;  // list entries, searches, bitmap words
   b_ref += 4ull * kout * rows_written;               // rows written (tid 0 holds the tile counts)
   b_own += 4ull * kout * rows_written;
   if (tid == 0) {
