@@ -645,7 +645,12 @@ def main():
                     help="comma-separated further arrival rates for the served workload ('' for none)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="profiling: the headline steps only")
+    ap.add_argument("--sections", type=str, default="c4,cpu,c2,batched,served",
+                    help="comma-separated sections to run (the headline c4 always runs unless omitted here)")
     args = ap.parse_args()
+    sections = set(x.strip() for x in args.sections.split(","))
+    if args.headline_only:
+        sections = {"c4"}
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; multi-GPU sharding is a design only "
                                    "(queries or root candidates split across GPUs, one u64 count reduction)"}))
@@ -654,6 +659,20 @@ def main():
     ctx = csm.Context(0, int(args.arena_gb * (1 << 30)))
     ms_copy = ctx.copy_probe(1 << 30, 10)
     copy_gbs = 2.0 * (1 << 30) / (ms_copy * 1e6)
+    if "c4" not in sections:  # partial runs for profiling and development
+        line = {"metric": METRIC, "partial": sorted(sections)}
+        if "c2" in sections:
+            line["c2_full"] = c2_section(ctx, [int(x) for x in args.c2_limits_ms.split(",")])
+        if "batched" in sections:
+            line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_item_budget,
+                                              args.batched_sample)
+        if "served" in sections:
+            more = [float(x) for x in args.served_more_qps.split(",") if x.strip()]
+            line["served"] = served_section(ctx, args.served_queries, args.served_qps, args.batched_cap,
+                                            args.batched_limit_ms, 64, 64, extra_qps=more)
+        ctx.close()
+        print(json.dumps(line), flush=True)
+        return
     c4 = c4_section(ctx, args)
     elapsed, tot = c4["elapsed"], c4["tot"]
     nv, ne, maxdeg = c4["graph"]
@@ -690,16 +709,16 @@ def main():
     }
     hg, hqs = c4["hg"], c4["hqs"]
     del c4
-    if not args.headline_only and not args.no_cpu:
+    if "cpu" in sections and not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hg, hqs[:4], C4_CAP, args.cpu_seconds)
     del hg, hqs
-    if not args.headline_only and not args.no_c2:
+    if "c2" in sections and not args.no_c2:
         lim = [int(x) for x in args.c2_limits_ms.split(",")]
         line["c2_full"] = c2_section(ctx, lim)
-    if not args.headline_only and not args.no_batched:
+    if "batched" in sections and not args.no_batched:
         line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_item_budget,
                                           args.batched_sample)
-    if not args.headline_only and not args.no_served:
+    if "served" in sections and not args.no_served:
         more = [float(x) for x in args.served_more_qps.split(",") if x.strip()]
         line["served"] = served_section(ctx, args.served_queries, args.served_qps, args.batched_cap,
                                         args.batched_limit_ms, 64, 64, extra_qps=more)

@@ -127,7 +127,7 @@ class BatchOpts(C.Structure):
 class ServeOpts(C.Structure):
     _fields_ = [("budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("cap", C.c_uint64), ("refine", C.c_uint32),
-                ("chunk_rows", C.c_uint32)]
+                ("chunk_rows", C.c_uint32), ("wgs_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class CtxInfo(C.Structure):
@@ -368,11 +368,11 @@ class Context:
         return DeviceGraph(self, h)
 
     def serve_start(self, graph, budget_bytes=64 << 30, max_inflight=64, queue_capacity=4096, cap=0,
-                    refine=False, chunk_rows=0):
+                    refine=False, chunk_rows=0, wgs_per_cu=0):
         """Continuous batching against `graph` (csm_serve_start); submit and
         poll from any thread, and no other matching call on this context
         until serve_stop."""
-        o = ServeOpts(budget_bytes, max_inflight, queue_capacity, cap, 1 if refine else 0, chunk_rows)
+        o = ServeOpts(budget_bytes, max_inflight, queue_capacity, cap, 1 if refine else 0, chunk_rows, wgs_per_cu, 0)
         self.check(lib().csm_serve_start(self.h, graph.h, C.byref(o)), "csm_serve_start")
 
     def serve_submit(self, plan):

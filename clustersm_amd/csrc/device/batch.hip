@@ -67,6 +67,9 @@ struct BState {
   unsigned long long pending;  // tasks of the query queued or running
   unsigned long long items;  // (row, list entry) pairs examined for the query
   unsigned long long rows;   // partial rows written for the query
+  unsigned int gen;          // resident mode: admission generation of the slot
+  unsigned int pad;
+  unsigned long long adm;    // resident mode: admission index of the slot's query
 };
 
 enum : unsigned int { kRunning = 0, kCapped = 1, kAbandoned = 2 };
@@ -82,6 +85,30 @@ struct alignas(16) Task {
 };
 
 constexpr uint32_t kNil = 0xFFFFFFFFu;
+
+// Resident (serving) mode.  The host publishes prepared queries through a
+// block of pinned, coherent host memory that the running kernel polls:
+// admission a names a slot and the slot's generation; `doorbell` counts the
+// admissions written.  The kernel initialises each admitted slot's state
+// itself (device atomics) before any of its tasks run, and writes a
+// completion record per slot, generation last, when the query's last task
+// ends.  Admissions in flight never exceed the slots, so kAdmRing > slots.
+constexpr uint32_t kAdmRing = 1024;
+constexpr uint32_t kMaxSlots = 256;
+struct SlotAdmit {
+  uint32_t slot, gen;
+};
+struct SlotDone {
+  unsigned long long count, items, rows;
+  unsigned int lost;
+  unsigned int gen;  // written last, with a release: the record is complete
+};
+struct ServeShared {
+  unsigned long long doorbell;  // admissions published by the host
+  unsigned long long stop;      // 1: the kernel exits once idle
+  SlotAdmit adm[kAdmRing];
+  SlotDone done[kMaxSlots];
+};
 constexpr unsigned long long kNilEntry = ~0ull;  // deque entries: record id | seq << 32
 constexpr uint32_t kDequeCap = 1024;  // > levels x (held blocks + 2): a deque never fills
 
@@ -134,6 +161,16 @@ struct BCtl {
   unsigned long long bad_t, bad_b;   // slot index and bound it was taken at
   unsigned int late_src, late_victim;  // the same for the first late record
   unsigned long long late_t, late_b;
+  // resident mode: admissions whose slots are initialised, the host doorbell
+  // and stop flag as last read (polled by one workgroup at a time, at most
+  // every few microseconds), and the slot of each admission
+  unsigned long long adm_seen;
+  unsigned long long db_mirror;
+  unsigned long long stop_mirror;
+  unsigned long long last_poll;
+  unsigned int adm_lock;
+  unsigned int exiting;            // resident mode: set once, under adm_lock; no admission follows
+  unsigned int adm_slot[kAdmRing];
 };
 
 struct BArgs {
@@ -159,6 +196,10 @@ struct BArgs {
   uint64_t budget;                 // items per query before it stops with a lower bound (the caller's budget)
   uint32_t cache_cap;              // free blocks a workgroup may keep (<= kCache)
   uint64_t deadline_ticks;         // s_memrealtime ticks (100 MHz) before the kernel gives up
+  uint32_t serve;                  // resident mode: queries arrive through `sh` while the kernel runs
+  uint32_t pad;
+  ServeShared *sh;                 // pinned host block (resident mode)
+  uint64_t idle_exit_ticks;        // resident mode: exit after this long with nothing to do anywhere
 };
 
 __device__ inline unsigned long long ld_rlx(unsigned long long *p) {
@@ -267,7 +308,8 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     st_init(&a.dq[i].top, 0ull);
     st_init(&a.dq[i].pub, 0ull);
   }
-  for (uint64_t i = tid; i < a.nqueries; i += stride) {
+  // resident mode: slot states are set when each query is admitted
+  for (uint64_t i = tid; i < (a.serve ? 0u : a.nqueries); i += stride) {
     st_init(&a.st[i].count, 0ull);
     st_init(&a.st[i].done, a.qs[i].active ? static_cast<unsigned int>(kRunning) : static_cast<unsigned int>(kCapped));
     st_init(&a.st[i].lost, 0u);
@@ -279,7 +321,16 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     st_init(a.free_head, 0ull);
     st_init(a.free_tail, static_cast<unsigned long long>(a.num_blocks));
     BCtl &c = *a.ctl;
-    unsigned long long *c64[] = {&c.root_next, &c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out,
+    if (!a.serve) {  // resident mode: root and admission progress carry over to a relaunch
+      st_init(&c.root_next, 0ull);
+      st_init(&c.adm_seen, 0ull);
+      st_init(&c.db_mirror, 0ull);
+    }
+    st_init(&c.stop_mirror, 0ull);
+    st_init(&c.last_poll, 0ull);
+    st_init(&c.adm_lock, 0u);
+    st_init(&c.exiting, 0u);
+    unsigned long long *c64[] = {&c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out,
                                  &c.bytes_ref, &c.bytes_own, &c.tasks,
                                  &c.steals, &c.fails, &c.busy_ticks, &c.idle_ticks, &c.sync_ticks, &c.phase_ticks[0],
                                  &c.phase_ticks[1], &c.phase_ticks[2], &c.phase_ticks[3], &c.alloc_fail,
@@ -324,6 +375,10 @@ struct Smem {
   uint32_t out_block;
   uint32_t rec;                     // record of the task being processed
   uint32_t staged_q, staged_k;      // query header and level descriptor held in LDS
+  uint32_t staged_gen, task_gen;    // resident mode: slot generation staged, and of the task in hand
+  unsigned long long task_adm;      // resident mode: admission index of the task's query
+  unsigned long long acq_seen;      // resident mode: admissions published before this workgroup's last invalidation
+  unsigned long long idle_since;    // resident mode: when this workgroup last found nothing to do
   uint32_t nheld;                   // full output blocks of the current task, in row order
   uint32_t held[kMaxOut];
   uint32_t held_fill[kMaxOut];
@@ -654,7 +709,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   for (; row0 < nrows; row0 += kRowBatch) {
     if (tid == 0) {
       sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
-      if (sm.stop && ld_done(a, t.q) == kRunning) a.st[t.q].lost = 1;  // cut short by an early stop
+      if (sm.stop && ld_done(a, t.q) == kRunning) __hip_atomic_store(&a.st[t.q].lost, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // cut short by an early stop
     }
     __syncthreads();
     if (sm.stop) break;
@@ -965,7 +1020,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   b_own += 4ull * kout * rows_written;
   if (tid == 0) {
     const bool live = ld_done(a, t.q) == kRunning && !aborted(a);
-    if (!live && ld_done(a, t.q) == kRunning) a.st[t.q].lost = 1;  // its outputs are dropped by an early stop
+    if (!live && ld_done(a, t.q) == kRunning) __hip_atomic_store(&a.st[t.q].lost, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // its outputs are dropped by an early stop
     uint32_t np = 0;
     if (sm.flag != 0 && live) {  // same rows, resume point updated
       sm.push[np++] = write_rec(a, sm, wg, sm.rec, sm.task);
@@ -1007,6 +1062,107 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   __syncthreads();
 }
 
+// ---------------------------------------------------------------- resident mode
+template <typename T>
+__device__ inline T ld_sys(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+constexpr uint64_t kPollTicks = 200;  // host doorbell read at most every 2 us (100 MHz clock)
+
+// Thread 0: reads the host doorbell (one workgroup at a time, rate-limited)
+// and initialises the slots of new admissions: state first, then the slot of
+// each admission, then adm_seen with a release, so a root task is only ever
+// taken for an initialised slot.
+__device__ void serve_admit(const BArgs &a) {
+  const unsigned long long now = __builtin_amdgcn_s_memrealtime();
+  const unsigned long long last = ld_rlx(&a.ctl->last_poll);
+  if (now - last < kPollTicks || atomicCAS(&a.ctl->last_poll, last, now) != last) return;
+  const unsigned long long db = ld_sys(&a.sh->doorbell);
+  st_rlx(&a.ctl->stop_mirror, ld_sys(&a.sh->stop));
+  st_rlx(&a.ctl->db_mirror, db);
+  if (db <= ld_rlx(&a.ctl->adm_seen) || atomicCAS(&a.ctl->adm_lock, 0u, 1u) != 0u) return;
+  acquire_agent();
+  if (__hip_atomic_load(&a.ctl->exiting, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {  // the next launch admits
+    __hip_atomic_store(&a.ctl->adm_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const unsigned long long s0 = ld_rlx(&a.ctl->adm_seen);
+  unsigned long long s = s0;
+  for (; s < db && s < s0 + 64; ++s) {
+    const SlotAdmit *e = &a.sh->adm[s % kAdmRing];
+    const uint32_t slot = __hip_atomic_load(&e->slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint32_t gen = __hip_atomic_load(&e->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (slot >= a.nqueries) {  // never expected: a host bug; stop, the host recounts
+      atomicCAS(&a.ctl->abort, 0u, kAbortBadTask);
+      break;
+    }
+    BState &q = a.st[slot];
+    st_init(&q.count, 0ull);
+    st_init(&q.done, static_cast<unsigned int>(kRunning));
+    st_init(&q.lost, 0u);
+    st_init(&q.pending, static_cast<unsigned long long>(a.tasks_per_query));
+    st_init(&q.items, 0ull);
+    st_init(&q.rows, 0ull);
+    st_init(&q.gen, gen);
+    st_init(&q.adm, s);
+    __hip_atomic_store(&a.ctl->adm_slot[s % kAdmRing], slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __hip_atomic_store(&a.ctl->adm_seen, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&a.ctl->adm_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0: n tasks of query q are finished; the query's last publishes its
+// completion record (the counts of finished tasks are in its state already).
+__device__ void serve_tasks_done(const BArgs &a, uint32_t q, unsigned long long n) {
+  const unsigned long long prev =
+      __hip_atomic_fetch_add(&a.st[q].pending, 0ull - n, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (prev != n) return;
+  BState &st = a.st[q];
+  SlotDone *d = &a.sh->done[q];
+  const unsigned int gen = __hip_atomic_load(&st.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&d->count, ld_rlx(&st.count), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&d->items, ld_rlx(&st.items), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&d->rows, ld_rlx(&st.rows), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&d->lost, __hip_atomic_load(&st.lost, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&d->gen, gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Thread 0: the next root task of an admitted query, if any: root i is range
+// i % T of admission i / T (T root tasks per query).  The remaining roots of
+// a query that is already finished (capped) are taken together and counted
+// as finished tasks, which keeps its task accounting exact.
+__device__ bool serve_take_root(const BArgs &a, Smem &sm, uint32_t &rec) {
+  const unsigned long long T = a.tasks_per_query;
+  for (;;) {
+    const unsigned long long seen = __hip_atomic_load(&a.ctl->adm_seen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long i = ld_rlx(&a.ctl->root_next);
+    if (i >= seen * T) return false;
+    const uint32_t slot =
+        __hip_atomic_load(&a.ctl->adm_slot[(i / T) % kAdmRing], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ld_done(a, slot) != kRunning) {
+      const unsigned long long end = (i / T + 1) * T;
+      if (atomicCAS(&a.ctl->root_next, i, end) == i) serve_tasks_done(a, slot, end - i);
+      continue;
+    }
+    if (atomicCAS(&a.ctl->root_next, i, i + 1) != i) continue;
+    const uint32_t c = static_cast<uint32_t>(i % T);
+    rec = static_cast<uint32_t>(a.num_blocks + static_cast<uint64_t>(slot) * T + c);
+    sm.task = Task{slot, 1, 0, c * a.root_words, min(a.words, (c + 1) * a.root_words), 0, 0, 0};
+    sm.task_stolen = 0;
+    return true;
+  }
+}
+
+// Thread 0, at the end of every task of query q in resident mode: the
+// workgroup's counts of q reach the query's state before the task leaves the
+// pending count.
+__device__ void serve_task_end(const BArgs &a, Smem &sm, uint32_t q) {
+  flush_acc(a, sm);
+  serve_tasks_done(a, q, 1);
+}
+
 __device__ inline uint32_t xorshift(uint32_t &s) {
   s ^= s << 13;
   s ^= s >> 17;
@@ -1037,6 +1193,9 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
   if (tid == 0) {
     sm.staged_q = kNil;
     sm.staged_k = kNil;
+    sm.staged_gen = sm.task_gen = 0;
+    sm.task_adm = sm.acq_seen = 0;
+    sm.idle_since = 0;
     sm.ncache = 0;
     sm.seq_ctr = 0;
     sm.dq_bottom = sm.dq_pub = 0;
@@ -1052,9 +1211,23 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       sm.state = 1;
       const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
       // own deque, then a new root task, then the oldest task of another workgroup
-      unsigned long long ent = idle ? kNilEntry : deque_pop(a, sm, wg);  // an idle workgroup's deque is empty
       uint32_t rec = kNil;
-      if (ent == kNilEntry && ld_rlx(&a.ctl->root_next) < nroot) {
+      // resident mode: a newly admitted query's roots come before this
+      // workgroup's own older tasks (while its deque holds few tasks and so
+      // few pool blocks), so new queries overtake a long one instead of
+      // queueing behind it
+      if (a.serve && sm.dq_bottom - ld_rlx(&a.dq[wg].top) < 16 && serve_take_root(a, sm, rec) && idle) {
+        atomicAdd(&a.ctl->idle, ~0ull);
+        idle = false;
+      }
+      unsigned long long ent = rec != kNil || idle ? kNilEntry : deque_pop(a, sm, wg);  // an idle deque is empty
+      if (rec != kNil) {
+      } else if (ent == kNilEntry && a.serve) {
+        if (serve_take_root(a, sm, rec) && idle) {
+          atomicAdd(&a.ctl->idle, ~0ull);
+          idle = false;
+        }
+      } else if (ent == kNilEntry && ld_rlx(&a.ctl->root_next) < nroot) {
         if (idle) {
           atomicAdd(&a.ctl->idle, ~0ull);
           idle = false;
@@ -1117,15 +1290,55 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
         sm.rec = rec;
         sm.state = 0;
         sm.stop = ld_done(a, sm.task.q) != kRunning || aborted(a);
+        sm.task_gen = a.serve ? __hip_atomic_load(&a.st[sm.task.q].gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        sm.task_adm = a.serve ? ld_rlx(&a.st[sm.task.q].adm) : 0ull;
+        if (a.serve && !sm.stop && (sm.staged_q != sm.task.q || sm.staged_gen != sm.task_gen) &&
+            sm.task_adm >= sm.acq_seen) {
+          // the query's descriptor, lists and bitmaps were written after this
+          // workgroup's last invalidation, over memory whose older contents
+          // this CU and XCD may still cache: invalidate once before anyone
+          // reads them (the barrier below orders the other threads' loads);
+          // admissions published later are read after adm_seen is read here
+          sm.acq_seen = ld_rlx(&a.ctl->adm_seen);
+          acquire_agent();
+        }
       } else {
+        const unsigned long long now = __builtin_amdgcn_s_memrealtime();
         if (!idle) {
           atomicAdd(&a.ctl->idle, 1ull);
           idle = true;
+          sm.idle_since = now;
         }
         // done when every workgroup that has begun is idle (idle read first: it
         // never exceeds `started`, so equality held at the moment it was read)
         if (aborted(a)) {
           sm.state = 2;
+        } else if (a.serve) {
+          // new admissions first; the whole kernel exits once everything is
+          // idle and either the host asked to stop or nothing arrived for a
+          // while (the host relaunches for later arrivals).  The decision is
+          // taken under the admission lock and stops admissions, so no work
+          // can appear after it: every workgroup exits at its next idle turn.
+          if (__hip_atomic_load(&a.ctl->exiting, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            sm.state = 2;
+          } else {
+            serve_admit(a);
+            const unsigned long long seen = ld_rlx(&a.ctl->adm_seen);
+            if (ld_rlx(&a.ctl->root_next) < seen * a.tasks_per_query) {
+              sm.idle_since = now;
+            } else if (now - sm.idle_since > a.idle_exit_ticks || ld_rlx(&a.ctl->stop_mirror)) {
+              if (atomicCAS(&a.ctl->adm_lock, 0u, 1u) == 0u) {
+                acquire_agent();
+                const unsigned long long seen2 = ld_rlx(&a.ctl->adm_seen);
+                if (ld_rlx(&a.ctl->idle) == ld_rlx(&a.ctl->started) && ld_rlx(&a.ctl->db_mirror) == seen2 &&
+                    ld_rlx(&a.ctl->root_next) >= seen2 * a.tasks_per_query) {
+                  __hip_atomic_store(&a.ctl->exiting, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  sm.state = 2;
+                }
+                __hip_atomic_store(&a.ctl->adm_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+              }
+            }
+          }
         } else if (ld_rlx(&a.ctl->root_next) >= nroot) {
           const unsigned long long nidle = ld_rlx(&a.ctl->idle);
           if (nidle == ld_rlx(&a.ctl->started)) sm.state = 2;
@@ -1147,7 +1360,9 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
     // a stolen task's rows were published by the victim's unlock and acquired
     // by the lock of the steal (thread 0); the barrier orders the other reads
     const uint32_t q = sm.task.q, k = sm.task.k;
-    if (sm.staged_q != q) {  // stage the query header (uniform branch)
+    // a dropped task (its query finished, or the kernel stops) needs nothing staged
+    const bool new_q = !sm.stop && (sm.staged_q != q || sm.staged_gen != sm.task_gen);
+    if (new_q) {  // stage the query header (uniform branch)
       const uint32_t *src = reinterpret_cast<const uint32_t *>(&a.qs[q]);
       uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.q);
       for (uint32_t i = tid; i < sizeof(BQuery) / 4; i += kThreads) dst[i] = src[i];
@@ -1158,29 +1373,35 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
         for (uint32_t i = tid; i < sizeof(LevelJoin) / 4; i += kThreads) d2[i] = s2[i];
       }
     }
-    if (sm.staged_q != q || sm.staged_k != k) {  // and the level descriptor
+    if (!sm.stop && (new_q || sm.staged_k != k)) {  // and the level descriptor
       const uint32_t *src = reinterpret_cast<const uint32_t *>(&sm.q.lv[k]);
       uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.lv);
       for (uint32_t i = tid; i < sizeof(LevelJoin) / 4; i += kThreads) dst[i] = src[i];
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0 && !sm.stop) {  // what LDS holds now
       sm.staged_q = q;
       sm.staged_k = k;
+      sm.staged_gen = sm.task_gen;
     }
     if (!sm.stop) {
       process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out, b_ref, b_own);
       if (sm.flag == 2)  // pool empty: give other workgroups time to free blocks
         for (int i = 0; i < 16; ++i) __builtin_amdgcn_s_sleep(32);
     } else if (tid == 0) {
-      if (ld_done(a, q) == kRunning) a.st[q].lost = 1;  // dropped by an early stop
+      if (ld_done(a, q) == kRunning) __hip_atomic_store(&a.st[q].lost, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // dropped by an early stop
       if (k >= 2) blk_free(a, sm, sm.task.block);
     }
     __syncthreads();
-    if (tid == 0) atomicAdd(&a.st[q].pending, ~0ull);  // this task is finished
+    if (tid == 0) {  // this task is finished
+      if (a.serve) serve_task_end(a, sm, q);
+      else atomicAdd(&a.st[q].pending, ~0ull);
+    }
     ++n_tasks;
     busy += __builtin_amdgcn_s_memrealtime() - t_task;
   }
+  if (a.serve && tid == 0)  // other workgroups may still run: give back the cached blocks
+    while (sm.ncache) free_push(a, sm.cache[--sm.ncache]);
   const unsigned long long wg_ref = block_reduce(b_ref, sm.red);
   const unsigned long long wg_own = block_reduce(b_own, sm.red);
   if (tid == 0) {
@@ -1201,6 +1422,8 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
 }
 
 uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(65536, (n + 255) / 256 + 1)); }
+
+std::atomic<uint32_t> g_launches{0};  // join launches of this process: part of every task record's tag
 
 struct TuneParams {
   uint32_t wgs_per_cu, root_words, block_kb, ilp, max_out;
@@ -1336,6 +1559,8 @@ csm_status BatchRun::prepare(hipStream_t s) {
   }
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_lv, lvs.data(), sizeof(LevelJoin) * lvs.size(), hipMemcpyHostToDevice, s));
   CSM_HIP_TRY(ctx, hipMemcpyAsync(d_bq_, bq.data(), sizeof(BQuery) * nqs, hipMemcpyHostToDevice, s));
+  bq_host_.resize(sizeof(BQuery) * nqs);
+  std::memcpy(bq_host_.data(), bq.data(), bq_host_.size());
   CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
   prepared_ = true;
   return CSM_OK;
@@ -1383,8 +1608,7 @@ csm_status BatchRun::launch(hipStream_t s) {
   a.recs = recs;
   a.dq = dq;
   a.dq_buf = dq_buf;
-  static std::atomic<uint32_t> launches{0};
-  a.epoch = launches.fetch_add(1) + 1;
+  a.epoch = g_launches.fetch_add(1) + 1;
   launch_epoch_ = a.epoch;
   tasks_per_query_ = tasks_per_query;
   a.nqueries = nqs;
@@ -1561,6 +1785,162 @@ csm_status BatchRun::finish(csm_result *results) {
     std::fprintf(stderr, "csm batch: %u queries finished by the plain path (%u with unclosed task accounting), %.1f ms in all\n",
                  fallbacks, unclosed, ms_all);
   return CSM_OK;
+}
+
+csm_status BatchRun::copy_descriptor(uint32_t i, void *dst, hipStream_t s) const {
+  if (!prepared_ || i >= queries_.size()) return set_error(ctx_, CSM_E_ARG, "descriptor of an unprepared query");
+  CSM_HIP_TRY(ctx_, hipMemcpyAsync(dst, bq_host_.data() + sizeof(BQuery) * i, sizeof(BQuery), hipMemcpyHostToDevice, s));
+  return CSM_OK;
+}
+
+// ------------------------------------------------------------ resident join
+ResidentJoin::ResidentJoin(csm_ctx *ctx, const csm_graph *g, uint32_t slots, uint64_t pool_bytes,
+                           uint32_t chunk_rows, uint32_t wgs_per_cu)
+    : ctx_(ctx), g_(g), slots_(std::min(std::max(slots, 1u), kMaxSlots)),
+      root_words_(chunk_rows ? std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, chunk_rows / 64)) : 16u),
+      wgs_per_cu_(wgs_per_cu ? std::min(wgs_per_cu, 4u) : 2u), pool_bytes_(pool_bytes), mem_(ctx) {}
+
+ResidentJoin::~ResidentJoin() {
+  shutdown();
+  if (ev_) hipEventDestroy(ev_);
+  delete static_cast<BArgs *>(args_);
+  if (sh_) hipHostFree(sh_);
+  mem_.release();
+}
+
+csm_status ResidentJoin::init(hipStream_t s) {
+  s_ = s;
+  csm_ctx *ctx = ctx_;
+  const uint32_t words = ceil_div(std::max<uint32_t>(g_->n, 1), 64);
+  const uint32_t T = ceil_div(words, root_words_);
+  const uint64_t block_words = 128ull * 1024 / 4;
+  const uint64_t num_blocks = std::max<uint64_t>(64, pool_bytes_ / (4 * block_words));
+  if (num_blocks + static_cast<uint64_t>(slots_) * T >= kNil) return set_error(ctx, CSM_E_LIMIT, "too many blocks");
+  const uint32_t grid = ctx->num_cus * wgs_per_cu_;
+  BArgs *a = new BArgs{};
+  args_ = a;
+  a->qs = static_cast<const BQuery *>(mem_.get(sizeof(BQuery) * slots_));
+  a->st = static_cast<BState *>(mem_.get(sizeof(BState) * slots_));
+  a->ctl = static_cast<BCtl *>(mem_.get(sizeof(BCtl)));
+  a->pool = static_cast<uint32_t *>(mem_.get(4 * block_words * num_blocks));
+  a->recs = static_cast<Task *>(mem_.get(sizeof(Task) * (num_blocks + static_cast<uint64_t>(slots_) * T)));
+  a->dq = static_cast<Deque *>(mem_.get(sizeof(Deque) * grid));
+  a->dq_buf = static_cast<unsigned long long *>(mem_.get(8ull * kDequeCap * grid));
+  a->free_ring = static_cast<FreeSlot *>(mem_.get(sizeof(FreeSlot) * num_blocks));
+  unsigned long long *ctr = static_cast<unsigned long long *>(mem_.get(16));
+  if (!a->qs || !a->st || !a->ctl || !a->pool || !a->recs || !a->dq || !a->dq_buf || !a->free_ring || !ctr)
+    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (resident join)");
+  CSM_HIP_TRY(ctx, hipHostMalloc(&sh_, sizeof(ServeShared), hipHostMallocCoherent));
+  std::memset(sh_, 0, sizeof(ServeShared));
+  CSM_HIP_TRY(ctx, hipMemsetAsync(a->ctl, 0, sizeof(BCtl), s));
+  CSM_HIP_TRY(ctx, hipMemsetAsync(const_cast<BQuery *>(a->qs), 0, sizeof(BQuery) * slots_, s));
+  CSM_HIP_TRY(ctx, hipMemsetAsync(a->st, 0, sizeof(BState) * slots_, s));
+  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  CSM_HIP_TRY(ctx, hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+  a->probe = AdjProbe{g_->off, g_->adj, g_->hub_id, g_->hub_bits, words};
+  a->nqueries = slots_;
+  a->tasks_per_query = T;
+  a->words = words;
+  a->root_words = root_words_;
+  a->free_head = ctr;
+  a->free_tail = ctr + 1;
+  a->num_blocks = num_blocks;
+  a->block_words = block_words;
+  a->ilp = kIlp;
+  a->max_out = 1;
+  a->budget = ~0ull;
+  a->cache_cap = static_cast<uint32_t>(std::min<uint64_t>(kCache, num_blocks / (4ull * grid)));
+  // a resident kernel stops after 120 s (its unfinished queries are then
+  // recounted on the plain path) and is relaunched for the rest
+  a->deadline_ticks = static_cast<uint64_t>(120.0 * 1e8);
+  a->serve = 1;
+  a->sh = static_cast<ServeShared *>(sh_);
+  a->idle_exit_ticks = 200000;  // 2 ms (100 MHz clock)
+  free_.clear();
+  for (uint32_t i = slots_; i-- > 0;) free_.push_back(i);
+  gen_.assign(slots_, 0);
+  busy_.assign(slots_, 0);
+  return CSM_OK;
+}
+
+csm_status ResidentJoin::launch() {
+  BArgs &a = *static_cast<BArgs *>(args_);
+  a.epoch = g_launches.fetch_add(1) + 1;
+  const uint32_t grid = ctx_->num_cus * wgs_per_cu_;
+  const size_t lds_rows = 4ull * kRowBatch * (CSM_MAX_QV - 1);
+  batch_init_kernel<<<grid_for(std::max<uint64_t>(a.num_blocks, grid)), 256, 0, s_>>>(a, grid);
+  batch_join_kernel<4><<<grid, kThreads, lds_rows, s_>>>(a);
+  CSM_HIP_TRY(ctx_, hipGetLastError());
+  CSM_HIP_TRY(ctx_, hipEventRecord(ev_, s_));
+  running_ = true;
+  return CSM_OK;
+}
+
+csm_status ResidentJoin::admit(const BatchRun &b, uint32_t i, hipStream_t s, uint32_t &slot) {
+  if (free_.empty()) return set_error(ctx_, CSM_E_BUSY, "no free slot");
+  slot = free_.back();
+  BArgs &a = *static_cast<BArgs *>(args_);
+  csm_status st = b.copy_descriptor(i, const_cast<BQuery *>(a.qs) + slot, s);
+  if (st != CSM_OK) return st;
+  CSM_HIP_TRY(ctx_, hipStreamSynchronize(s));
+  free_.pop_back();
+  busy_[slot] = 1;
+  const uint32_t gen = ++gen_[slot];
+  ServeShared *sh = static_cast<ServeShared *>(sh_);
+  SlotAdmit &e = sh->adm[doorbell_ % kAdmRing];
+  __atomic_store_n(&e.slot, slot, __ATOMIC_RELAXED);
+  __atomic_store_n(&e.gen, gen, __ATOMIC_RELAXED);
+  __atomic_store_n(&sh->doorbell, ++doorbell_, __ATOMIC_RELEASE);
+  return CSM_OK;
+}
+
+csm_status ResidentJoin::poll(std::vector<Done> &done) {
+  ServeShared *sh = static_cast<ServeShared *>(sh_);
+  for (uint32_t slot = 0; slot < slots_; ++slot) {
+    if (!busy_[slot] || __atomic_load_n(&sh->done[slot].gen, __ATOMIC_ACQUIRE) != gen_[slot]) continue;
+    const SlotDone &d = sh->done[slot];
+    done.push_back(Done{slot, __atomic_load_n(&d.count, __ATOMIC_RELAXED), __atomic_load_n(&d.items, __ATOMIC_RELAXED),
+                        __atomic_load_n(&d.rows, __ATOMIC_RELAXED), __atomic_load_n(&d.lost, __ATOMIC_RELAXED) != 0});
+    busy_[slot] = 0;
+    free_.push_back(slot);
+  }
+  if (running_ && hipEventQuery(ev_) == hipSuccess) {
+    running_ = false;
+    BArgs &a = *static_cast<BArgs *>(args_);
+    BCtl hctl{};
+    CSM_HIP_TRY(ctx_, hipMemcpyAsync(&hctl, a.ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s_));
+    CSM_HIP_TRY(ctx_, hipStreamSynchronize(s_));
+    if (hctl.abort) {
+      // stopped early: every admitted, unfinished query is recounted by the
+      // caller, and the next launch starts after the last admission
+      std::fprintf(stderr, "csm serve: resident join stopped early (abort %u, %llu late, %llu bad records)\n",
+                   hctl.abort, hctl.late_recs, hctl.bad_recs);
+      for (uint32_t slot = 0; slot < slots_; ++slot) {
+        if (!busy_[slot]) continue;
+        done.push_back(Done{slot, 0, 0, 0, true});
+        busy_[slot] = 0;
+        free_.push_back(slot);
+      }
+      hctl.root_next = doorbell_ * a.tasks_per_query;
+      hctl.adm_seen = hctl.db_mirror = doorbell_;
+      CSM_HIP_TRY(ctx_, hipMemcpyAsync(&a.ctl->root_next, &hctl.root_next, 8, hipMemcpyHostToDevice, s_));
+      CSM_HIP_TRY(ctx_, hipMemcpyAsync(&a.ctl->adm_seen, &hctl.adm_seen, 8, hipMemcpyHostToDevice, s_));
+      CSM_HIP_TRY(ctx_, hipMemcpyAsync(&a.ctl->db_mirror, &hctl.db_mirror, 8, hipMemcpyHostToDevice, s_));
+      CSM_HIP_TRY(ctx_, hipStreamSynchronize(s_));
+    }
+  }
+  bool waiting = false;
+  for (uint32_t slot = 0; slot < slots_ && !waiting; ++slot) waiting = busy_[slot] != 0;
+  if (!running_ && waiting) return launch();
+  return CSM_OK;
+}
+
+void ResidentJoin::shutdown() {
+  if (sh_) __atomic_store_n(&static_cast<ServeShared *>(sh_)->stop, 1ull, __ATOMIC_RELEASE);
+  if (running_) {
+    hipEventSynchronize(ev_);
+    running_ = false;
+  }
 }
 
 csm_status run_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<csm_query *> &queries,

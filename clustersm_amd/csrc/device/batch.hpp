@@ -26,6 +26,14 @@ class BatchRun {
   csm_status finish(csm_result *results);
 
   const std::vector<csm_query *> &queries() const { return queries_; }
+  // After prepare: whether query i needs the join, and its count when not
+  // (answered on the host: an empty root candidate set or two vertices)
+  bool active(uint32_t i) const { return active_[i] != 0; }
+  uint64_t host_count(uint32_t i) const { return host_count_[i]; }
+  // Copies query i's join descriptor (which points into this batch's memory)
+  // to device address dst on stream s.
+  csm_status copy_descriptor(uint32_t i, void *dst, hipStream_t s) const;
+  double ms_filter() const { return ms_filter_; }
 
  private:
   void *get(uint64_t bytes);
@@ -46,6 +54,7 @@ class BatchRun {
   std::vector<CandIndexHost> cih_;
   std::vector<uint8_t> active_;
   std::vector<uint64_t> host_count_;
+  std::vector<uint8_t> bq_host_;  // the join descriptors, host copy
   void *d_bq_ = nullptr;   // per-query join descriptors (device)
   void *d_st_ = nullptr;   // per-query counts and state (device)
   void *d_ctl_ = nullptr;  // scheduler counters (device)
@@ -53,6 +62,52 @@ class BatchRun {
   hipGraph_t graph_ = nullptr;             // captured init + join, alive until its launch has completed
   hipGraphExec_t graph_exec_ = nullptr;
   bool prepared_ = false, launched_ = false;
+};
+
+// The persistent join kept resident for serving: queries prepared by a
+// BatchRun are admitted into slots while the kernel runs (pinned doorbell),
+// each query's result is published as soon as its last task ends, and the
+// kernel exits when idle and is relaunched for later admissions.
+class ResidentJoin {
+ public:
+  struct Done {
+    uint32_t slot;
+    uint64_t count, items, rows;
+    bool recount;  // the kernel stopped early with the query unfinished: recount it exactly
+  };
+  ResidentJoin(csm_ctx *ctx, const csm_graph *g, uint32_t slots, uint64_t pool_bytes, uint32_t chunk_rows,
+               uint32_t wgs_per_cu);
+  ~ResidentJoin();
+  ResidentJoin(const ResidentJoin &) = delete;
+  ResidentJoin &operator=(const ResidentJoin &) = delete;
+  csm_status init(hipStream_t join_stream);
+  uint32_t free_slots() const { return static_cast<uint32_t>(free_.size()); }
+  // Admits active query i of a prepared batch: descriptor to a free slot
+  // (copied on stream s, which is synchronised), then the doorbell.
+  csm_status admit(const BatchRun &b, uint32_t i, hipStream_t s, uint32_t &slot);
+  // Launches the kernel when admitted work waits and it is not running;
+  // collects completed slots (their slots become free).
+  csm_status poll(std::vector<Done> &done);
+  // Asks the kernel to exit once idle and waits for it.
+  void shutdown();
+  bool running() const { return running_; }
+
+ private:
+  csm_status launch();
+  csm_ctx *ctx_;
+  const csm_graph *g_;
+  uint32_t slots_, root_words_, wgs_per_cu_;
+  uint64_t pool_bytes_;
+  WorkMem mem_;
+  hipStream_t s_ = nullptr;
+  hipEvent_t ev_ = nullptr;
+  bool running_ = false;
+  void *sh_ = nullptr;  // pinned ServeShared
+  void *args_ = nullptr;  // host copy of the kernel arguments
+  uint64_t doorbell_ = 0;
+  std::vector<uint32_t> free_;
+  std::vector<uint32_t> gen_;
+  std::vector<uint8_t> busy_;
 };
 
 // prepare + launch + finish on ctx->stream.

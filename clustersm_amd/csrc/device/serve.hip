@@ -3,28 +3,31 @@
 // The reference has no server; it runs one process per query
 // (runClusterSM.sh:9-13), re-reading and re-uploading the data graph each
 // time (main.cu:55-76).  Here the CSR, the neighbour-label table and the hub
-// bitmaps stay on the device, and one worker thread runs a two-stage
-// pipeline over two streams:
-//
-//   while batch k's persistent join kernel runs on the join stream, the
-//   requests that arrived meanwhile are admitted (in arrival order, up to
-//   max_inflight, while their estimated footprint fits the budget) and
-//   batch k+1's filter and candidate indexes are built on the prep stream;
-//   then batch k's results are published and batch k+1's join is launched.
+// bitmaps stay on the device, and one persistent join kernel stays resident
+// (ResidentJoin): a worker thread admits arriving requests in arrival order
+// while slots and the memory budget allow, builds their filter and candidate
+// indexes on a preparation stream beside the running kernel, and rings a
+// doorbell in pinned memory; the running kernel takes the new queries' root
+// tasks between its other tasks, so their work overlaps with the queries
+// already in flight, and publishes each query's result as soon as its last
+// task ends.  A query is never held back by the others admitted with it.
 //
 // Admission estimate per query: its candidate bitmaps plus, for every query
 // edge (s, t), |label(s)| * (8 + 4 * avg_degree * share(label(t))) bytes of
 // candidate-neighbour lists: the expected index size before filtering.  A
-// batch whose build still runs out of memory is split in halves and retried.
-// Each batch also holds a frontier pool of min(budget / 4, 8 GiB).
+// group whose build runs out of memory is split in halves and retried.  The
+// resident kernel's frontier pool is min(budget / 4, 8 GiB).
 //
 // Threading: submit/poll/stop are safe from any thread; while a server runs,
 // its worker owns the context's streams, so the context must not be used for
 // other matching calls until csm_serve_stop.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <list>
 #include <thread>
 #include <unordered_map>
 
@@ -54,17 +57,26 @@ class Server {
     Clock::time_point arrival;
     uint64_t est;
   };
-  struct Batch {
+  // Requests prepared together; their memory is released when the last one
+  // has its result.
+  struct Group {
     std::vector<Req> reqs;
     std::vector<csm_query *> queries;
     std::unique_ptr<BatchRun> run;
+    uint32_t outstanding = 0;
+    uint64_t est = 0;
+  };
+  struct Owner {
+    Group *g = nullptr;
+    uint32_t i = 0;
   };
 
   void loop();
-  std::vector<Req> admit(bool block);
-  std::unique_ptr<Batch> build(std::vector<Req> reqs);
-  void complete(Batch &b, csm_status st, const csm_result *res);
-  void release(Batch &b);
+  std::vector<Req> admit();
+  bool build(std::vector<Req> reqs);
+  void publish(const Req &r, csm_result res);
+  void finish_one(Group *g, uint32_t i, csm_result res);
+  void release(Group *g);
   uint64_t estimate(const csm_query_plan &p) const;
 
   csm_ctx *ctx_;
@@ -75,6 +87,10 @@ class Server {
   std::vector<uint64_t> label_count_;  // data vertices per label
   double avg_degree_ = 0;
   uint64_t pool_bytes_ = 0;
+  std::unique_ptr<ResidentJoin> rj_;
+  std::vector<Owner> owner_;               // per slot
+  std::list<std::unique_ptr<Group>> groups_;
+  uint64_t inflight_est_ = 0;
 
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
@@ -114,13 +130,20 @@ csm_status Server::start() {
   avg_degree_ = g_->n ? 2.0 * static_cast<double>(g_->m) / g_->n : 0.0;
   if (!o_.budget_bytes) o_.budget_bytes = std::min<uint64_t>(ctx_->arena.largest_free() / 2, 64ull << 30);
   if (!o_.max_inflight) o_.max_inflight = 256;
+  o_.max_inflight = std::min<uint32_t>(o_.max_inflight, 256);
   if (!o_.queue_capacity) o_.queue_capacity = 1u << 16;
   pool_bytes_ = std::min<uint64_t>(o_.budget_bytes / 4, 8ull << 30);
   bo_.cap = o_.cap;
   bo_.refine = o_.refine;
-  bo_.use_graph = 1;
+  bo_.use_graph = 0;
   bo_.chunk_rows = o_.chunk_rows;
-  bo_.pool_bytes = pool_bytes_;
+  rj_.reset(new ResidentJoin(ctx_, g_, o_.max_inflight, pool_bytes_, o_.chunk_rows, o_.wgs_per_cu));
+  csm_status st = rj_->init(s_join_);
+  if (st != CSM_OK) {
+    rj_.reset();
+    return st;
+  }
+  owner_.assign(o_.max_inflight, Owner{});
   running_ = true;
   worker_ = std::thread([this] { loop(); });
   return CSM_OK;
@@ -170,23 +193,23 @@ void Server::stop() {
     running_ = false;
     cv_done_.notify_all();
   }
+  rj_.reset();
   if (s_prep_) hipStreamDestroy(s_prep_);
   if (s_join_) hipStreamDestroy(s_join_);
   s_prep_ = s_join_ = nullptr;
 }
 
-// Requests for the next batch, in arrival order.  Blocks only when nothing
-// is running (`block`); always takes at least one request when any waits.
-std::vector<Server::Req> Server::admit(bool block) {
-  std::unique_lock<std::mutex> lk(mu_);
-  if (block)
-    while (pending_.empty() && !stopping_) cv_work_.wait(lk);
+// Requests to admit now, in arrival order, while slots and the budget allow
+// (the first always when nothing is in flight).
+std::vector<Server::Req> Server::admit() {
+  std::lock_guard<std::mutex> lk(mu_);
   std::vector<Req> out;
   const uint64_t limit = o_.budget_bytes / 2 > pool_bytes_ ? o_.budget_bytes / 2 - pool_bytes_ : 0;
-  uint64_t sum = 0;
-  while (!pending_.empty() && out.size() < o_.max_inflight) {
+  uint64_t sum = inflight_est_;
+  const uint32_t slots = rj_->free_slots();
+  while (!pending_.empty() && out.size() < slots) {
     const Req &r = pending_.front();
-    if (!out.empty() && sum + r.est > limit) break;
+    if ((!out.empty() || !groups_.empty()) && sum + r.est > limit) break;
     sum += r.est;
     out.push_back(r);
     pending_.pop_front();
@@ -194,92 +217,163 @@ std::vector<Server::Req> Server::admit(bool block) {
   return out;
 }
 
-void Server::release(Batch &b) {
-  b.run.reset();
-  for (csm_query *q : b.queries) csm_query_free(ctx_, q);
-  b.queries.clear();
-}
-
-void Server::complete(Batch &b, csm_status st, const csm_result *res) {
-  const Clock::time_point now = Clock::now();
+void Server::publish(const Req &r, csm_result res) {
+  res.ms_latency = std::chrono::duration<double, std::milli>(Clock::now() - r.arrival).count();
   std::lock_guard<std::mutex> lk(mu_);
-  for (size_t i = 0; i < b.reqs.size(); ++i) {
-    csm_result r{};
-    if (st == CSM_OK && res) r = res[i];
-    r.status = st;
-    r.ms_latency = std::chrono::duration<double, std::milli>(now - b.reqs[i].arrival).count();
-    done_[b.reqs[i].ticket] = r;
-  }
+  done_[r.ticket] = res;
   cv_done_.notify_all();
 }
 
-// Prepares the queries and the batch's filter and indexes on the prep
-// stream.  Out of memory: the batch is split and each half built alone
-// (the second half goes back to the front of the queue).
-std::unique_ptr<Server::Batch> Server::build(std::vector<Req> reqs) {
+void Server::release(Group *g) {
+  g->run.reset();
+  for (csm_query *q : g->queries) csm_query_free(ctx_, q);
+  g->queries.clear();
+  inflight_est_ -= g->est;
+  for (auto it = groups_.begin(); it != groups_.end(); ++it)
+    if (it->get() == g) {
+      groups_.erase(it);
+      break;
+    }
+}
+
+void Server::finish_one(Group *g, uint32_t i, csm_result res) {
+  publish(g->reqs[i], res);
+  if (--g->outstanding == 0) release(g);
+}
+
+// Prepares the requests' queries, filter and candidate indexes on the
+// preparation stream (beside the running kernel) and admits them.  Out of
+// memory: the group is split and the second half goes back to the queue.
+bool Server::build(std::vector<Req> reqs) {
   while (!reqs.empty()) {
-    std::unique_ptr<Batch> b(new Batch());
-    b->reqs = reqs;
+    std::unique_ptr<Group> g(new Group());
+    g->reqs = reqs;
     csm_status st = CSM_OK;
     {
       hipStream_t keep = ctx_->stream;
       ctx_->stream = s_prep_;
-      for (Req &r : b->reqs) {
+      for (Req &r : g->reqs) {
         csm_query *q = nullptr;
         st = csm_query_prepare(ctx_, g_, &r.plan, &q);
         if (st != CSM_OK) break;
-        b->queries.push_back(q);
+        g->queries.push_back(q);
       }
       ctx_->stream = keep;
     }
     if (st == CSM_OK) {
-      b->run.reset(new BatchRun(ctx_, g_, b->queries, bo_));
-      st = b->run->prepare(s_prep_);
+      g->run.reset(new BatchRun(ctx_, g_, g->queries, bo_));
+      st = g->run->prepare(s_prep_);
     }
-    if (st == CSM_OK) return b;
-    release(*b);
     if (st == CSM_E_NOMEM && reqs.size() > 1) {
+      release(g.get());
       std::vector<Req> back(reqs.begin() + reqs.size() / 2, reqs.end());
       reqs.resize(reqs.size() / 2);
       std::lock_guard<std::mutex> lk(mu_);
       for (auto it = back.rbegin(); it != back.rend(); ++it) pending_.push_front(*it);
       continue;
     }
-    Batch failed;
-    failed.reqs = reqs;
-    complete(failed, st, nullptr);
-    return nullptr;
+    Group *gp = g.get();
+    for (const Req &r : reqs) gp->est += r.est;
+    inflight_est_ += gp->est;
+    groups_.push_back(std::move(g));
+    gp->outstanding = static_cast<uint32_t>(reqs.size());
+    if (st != CSM_OK) {  // the whole group fails with the status
+      for (uint32_t i = 0; i < reqs.size(); ++i) {
+        csm_result r{};
+        r.status = st;
+        finish_one(gp, i, r);
+      }
+      return true;
+    }
+    for (uint32_t i = 0; i < reqs.size(); ++i) {
+      csm_result r{};
+      r.ms_filter = gp->run->ms_filter();
+      if (!gp->run->active(i)) {  // answered without a join
+        const uint64_t cap = o_.cap ? o_.cap : ~0ull;
+        r.count = std::min(gp->run->host_count(i), cap);
+        r.capped = gp->run->host_count(i) > cap;
+        r.path = CSM_PATH_HOST;
+        finish_one(gp, i, r);
+        continue;
+      }
+      uint32_t slot = 0;
+      st = rj_->admit(*gp->run, i, s_prep_, slot);
+      if (st != CSM_OK) {
+        r.status = st;
+        finish_one(gp, i, r);
+        continue;
+      }
+      owner_[slot] = Owner{gp, i};
+    }
+    return true;
   }
-  return nullptr;
+  return false;
 }
 
 void Server::loop() {
   hipSetDevice(ctx_->device);
-  std::unique_ptr<Batch> cur;  // launched on the join stream
+  const uint64_t cap = o_.cap ? o_.cap : ~0ull;
+  const char *tr = std::getenv("CSM_TRACE");
+  const bool trace = tr && *tr == '1';
+  const Clock::time_point t_start = Clock::now();
+  auto ms = [&]() { return std::chrono::duration<double, std::milli>(Clock::now() - t_start).count(); };
+  std::vector<ResidentJoin::Done> done;
   for (;;) {
-    std::vector<Req> reqs = admit(!cur);
-    std::unique_ptr<Batch> next = reqs.empty() ? nullptr : build(std::move(reqs));
-    if (cur) {
-      std::vector<csm_result> res(cur->reqs.size());
-      const csm_status st = cur->run->finish(res.data());
-      complete(*cur, st, res.data());
-      release(*cur);
-      cur.reset();
+    bool worked = false;
+    std::vector<Req> reqs = admit();
+    if (!reqs.empty()) {
+      const double t0 = ms();
+      const size_t n = reqs.size();
+      worked |= build(std::move(reqs));
+      if (trace) std::fprintf(stderr, "csm serve %.2f ms: %zu admitted, prepared in %.2f ms\n", ms(), n, ms() - t0);
     }
-    if (next) {
-      const csm_status st = next->run->launch(s_join_);
-      if (st != CSM_OK) {
-        complete(*next, st, nullptr);
-        release(*next);
-      } else {
-        cur = std::move(next);
+    done.clear();
+    const bool was_running = rj_->running();
+    csm_status st = rj_->poll(done);
+    if (trace && (done.size() || was_running != rj_->running()))
+      std::fprintf(stderr, "csm serve %.2f ms: %zu done, kernel %s\n", ms(), done.size(),
+                   rj_->running() ? (was_running ? "running" : "launched") : "stopped");
+    for (const ResidentJoin::Done &d : done) {
+      const Owner o = owner_[d.slot];
+      owner_[d.slot] = Owner{};
+      csm_result r{};
+      r.ms_filter = o.g->run->ms_filter();
+      r.count = std::min(d.count, cap);
+      r.capped = d.count > cap;
+      r.items = d.items;
+      r.partial_rows = d.rows;
+      r.path = CSM_PATH_BATCH;
+      if (d.recount) {  // the resident kernel stopped early: the plain path, exact
+        csm_match_opts mo{};
+        mo.cap = o_.cap;
+        mo.refine = o_.refine;
+        mo.reuse_filter = 1;
+        csm_result pr{};
+        const csm_status rs = run_join(ctx_, g_, o.g->queries[o.i], mo, pr);
+        r.count = pr.count;
+        r.capped = pr.capped;
+        r.items = pr.items;
+        r.status = rs;
+        r.path = CSM_PATH_BATCH_ABORT;
+      }
+      finish_one(o.g, o.i, r);
+      worked = true;
+    }
+    if (st != CSM_OK) {  // the kernel cannot be launched: fail what is in flight
+      for (uint32_t slot = 0; slot < owner_.size(); ++slot) {
+        if (!owner_[slot].g) continue;
+        csm_result r{};
+        r.status = st;
+        const Owner o = owner_[slot];
+        owner_[slot] = Owner{};
+        finish_one(o.g, o.i, r);
       }
     }
-    if (!cur) {
-      std::lock_guard<std::mutex> lk(mu_);
-      if (stopping_ && pending_.empty()) break;
-    }
+    std::unique_lock<std::mutex> lk(mu_);
+    if (stopping_ && pending_.empty() && groups_.empty()) break;
+    if (!worked) cv_work_.wait_for(lk, std::chrono::microseconds(50));
   }
+  rj_->shutdown();
 }
 
 }  // namespace csm

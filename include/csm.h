@@ -324,6 +324,9 @@ typedef struct csm_serve_opts {
   uint64_t cap;            /* per-query cap */
   uint32_t refine;
   uint32_t chunk_rows;
+  uint32_t wgs_per_cu;     /* resident join workgroups per CU (1..4), 0 = 2: the rest of each CU runs the
+                              arriving queries' filter and index builds beside it */
+  uint32_t reserved;
 } csm_serve_opts;
 csm_status csm_serve_start(csm_ctx *ctx, const csm_graph *g, const csm_serve_opts *opts);
 csm_status csm_serve_submit(csm_ctx *ctx, const csm_query_plan *plan, uint64_t *ticket);

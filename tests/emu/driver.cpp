@@ -4,7 +4,8 @@
 // ("noalloc ..."), then its list under a cap of 64
 // ("list <rows>" and one "row ..." line per embedding, sorted, query-id
 // order), "batch <count> <capped>", and with the candidate-aware order
-// (reorder option) "plainr ..." and "batchr ...".
+// (reorder option) "plainr ..." and "batchr ...", then through the server
+// ("served ...").
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -89,6 +90,21 @@ int main(int argc, char **argv) {
     std::printf("plainr %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
     csm_query_free(ctx, q);
   }
+  // served: the resident join, every query submitted, results polled
+  csm_serve_opts so{};
+  so.budget_bytes = 256ull << 20;
+  so.max_inflight = 4;  // fewer slots than queries: slots are reused
+  so.cap = cap;
+  check(csm_serve_start(ctx, g, &so), "serve start", ctx);
+  std::vector<uint64_t> tickets(plans.size());
+  for (size_t i = 0; i < plans.size(); ++i) check(csm_serve_submit(ctx, &plans[i], &tickets[i]), "submit", ctx);
+  for (size_t i = 0; i < plans.size(); ++i) {
+    csm_result r{};
+    check(csm_serve_poll(ctx, tickets[i], 1, &r), "serve poll", ctx);
+    if (r.status != CSM_OK) check(static_cast<csm_status>(r.status), "served result", ctx);
+    std::printf("served %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+  }
+  check(csm_serve_stop(ctx), "serve stop", ctx);
   for (csm_query *q : qs) csm_query_free(ctx, q);
   csm_graph_free(ctx, g);
   csm_host_graph_free(hd);

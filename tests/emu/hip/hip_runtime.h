@@ -32,6 +32,8 @@ enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDevice
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
 #define hipStreamNonBlocking 1
 #define hipHostMallocDefault 0
+#define hipHostMallocCoherent 0x40000000
+#define hipEventDisableTiming 0x2
 typedef struct EmuStream *hipStream_t;
 typedef struct EmuEvent *hipEvent_t;
 typedef struct EmuGraph *hipGraph_t;
@@ -70,6 +72,8 @@ hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipDeviceSynchronize();
 hipError_t hipEventCreate(hipEvent_t *e);
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }  // launches complete before they return
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
@@ -212,6 +216,7 @@ inline T atomicMax(T *p, U v) {
 #define __HIP_MEMORY_SCOPE_SYSTEM 4
 #define __hip_atomic_load(p, o, s) __atomic_load_n((p), __ATOMIC_SEQ_CST)
 #define __hip_atomic_store(p, v, o, s) __atomic_store_n((p), (v), __ATOMIC_SEQ_CST)
+#define __hip_atomic_fetch_add(p, v, o, s) __atomic_fetch_add((p), (v), __ATOMIC_SEQ_CST)
 
 template <class A, class B>
 inline typename std::common_type<A, B>::type min(A a, B b) {
