@@ -26,7 +26,7 @@ STATUS = {
     11: "result pending", 12: "graph exceeds the uint32 CSR layout", 13: "time limit reached",
     14: "item budget reached",
 }
-E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING, E_TIMEOUT, E_BUDGET = 2, 3, 4, 7, 8, 9, 11, 13, 14
+E_IO, E_FORMAT, E_QUERY, E_NOMEM, E_NO_GPU, E_BUSY, E_PENDING, E_LIMIT, E_TIMEOUT, E_BUDGET = 2, 3, 4, 7, 8, 9, 11, 12, 13, 14
 
 K_FILTER, K_REFINE, K_EXPAND, K_MATERIALIZE, K_COUNT, K_AUX, K_BATCH = range(7)
 K_NUM = 7
@@ -101,7 +101,7 @@ class MatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("reuse_filter", C.c_uint32),
                 ("list", C.POINTER(C.c_uint32)), ("list_capacity", C.c_uint64),
                 ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reorder", C.c_uint32),
-                ("item_budget", C.c_uint64)]
+                ("item_budget", C.c_uint64), ("symmetry", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -109,7 +109,7 @@ class Result(C.Structure):
                 ("listed", C.c_uint64), ("partial_rows", C.c_uint64), ("items", C.c_uint64),
                 ("ms_filter", C.c_double), ("ms_join", C.c_double), ("ms_total", C.c_double),
                 ("ms_latency", C.c_double), ("path", C.c_uint32), ("anomalies", C.c_uint32),
-                ("kernels", KernelStat * K_NUM)]
+                ("kernels", KernelStat * K_NUM), ("automorphisms", C.c_uint64)]
 
     def kernel(self, cls):
         k = self.kernels[cls]
@@ -168,6 +168,7 @@ SIGNATURES = {
     "csm_filter": ([_vp, _vp, _vp, C.c_int, _u64p], C.c_int),
     "csm_candidates_download": ([_vp, _vp, _u64p], C.c_int),
     "csm_query_reorder": ([_vp, _vp, _vp, C.c_int], C.c_int),
+    "csm_query_symmetry": ([C.POINTER(QueryPlan), _u64p, _u32p, _u32p], C.c_int),
     "csm_match": ([_vp, _vp, _vp, C.POINTER(MatchOpts), C.POINTER(Result)], C.c_int),
     "csm_match_noalloc": ([_vp, _vp, _vp, C.POINTER(MatchOpts), C.POINTER(Result)], C.c_int),
     "csm_batch_match": ([_vp, _vp, C.POINTER(_vp), C.c_uint32, C.POINTER(BatchOpts), C.POINTER(Result)], C.c_int),
@@ -282,6 +283,15 @@ class HostGraph:
         p = QueryPlan()
         _host_check(lib().csm_plan_build(self.h, C.byref(p)))
         return p
+
+
+def query_symmetry(plan):
+    """(|Aut(Q)|, [(v, w), ...] with f(v) < f(w)) for a QueryPlan (csm_query_symmetry);
+    raises CsmError(E_LIMIT) when the automorphism search is too costly."""
+    aut, n = C.c_uint64(), C.c_uint32()
+    pairs = (C.c_uint32 * (2 * MAX_QV * MAX_QV))()
+    _host_check(lib().csm_query_symmetry(C.byref(plan), C.byref(aut), C.byref(n), pairs))
+    return aut.value, [(pairs[2 * i], pairs[2 * i + 1]) for i in range(n.value)]
 
 
 def gen_erdos_renyi(n, m, labels, seed, zipf_s=0.0):
@@ -455,7 +465,7 @@ class Query:
                              "csm_query_reorder")
 
     def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0,
-              time_limit_ms=0, reorder=False, item_budget=0):
+              time_limit_ms=0, reorder=False, item_budget=0, symmetry=False):
         """Returns (Result, rows or None); rows are sorted, in query-id order.
         With time_limit_ms (item_budget), Result.status == E_TIMEOUT (E_BUDGET)
         marks a stopped search whose count is a lower bound.
@@ -464,6 +474,7 @@ class Query:
         o.cap, o.refine, o.reuse_filter, o.chunk_items = cap, 1 if refine else 0, 1 if reuse_filter else 0, chunk_items
         o.time_limit_ms = time_limit_ms
         o.item_budget = item_budget
+        o.symmetry = 1 if symmetry else 0
         o.reorder = 1 if reorder else 0
         rows = None
         if want_list:

@@ -31,6 +31,7 @@
 #include <numeric>
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "../host/host_graph.hpp"
 #include "cand_index.hpp"
 #include "filter.hpp"
 
@@ -40,6 +41,13 @@ namespace {
 
 constexpr int kTile = 256;          // items per workgroup
 constexpr int kTileWaves = kTile / kWave;
+
+// Symmetry-breaking conditions at one position: the new vertex must exceed
+// the row's ids at the `gt` positions and stay below those at `lt`.
+struct LevelSym {
+  uint32_t ngt, nlt;
+  uint32_t gt[CSM_MAX_QV], lt[CSM_MAX_QV];
+};
 
 struct ExpandArgs {
   AdjProbe probe;
@@ -66,6 +74,7 @@ struct ExpandArgs {
   const uint32_t *flen;
   uint32_t fused_rows;
   uint32_t last_inj_k;      // the last position's label equals position k's
+  LevelSym sym;             // symmetry-breaking conditions at position k (MARK / COUNT)
 };
 
 enum { kMark = 0, kCount = 1, kMaterialize = 2, kFused = 3 };
@@ -167,6 +176,8 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
     } else {
       ok = true;  // w is a candidate of u_k adjacent to the driver anchor
       for (uint32_t t = 0; ok && t < a.lv.ninj; ++t) ok = row[a.lv.inj[t]] != w;
+      for (uint32_t t = 0; ok && t < a.sym.ngt; ++t) ok = w > row[a.sym.gt[t]];
+      for (uint32_t t = 0; ok && t < a.sym.nlt; ++t) ok = w < row[a.sym.lt[t]];
       for (uint32_t t = 0; ok && t < a.lv.nanchor; ++t)
         if (t != d) ok = a.probe.has(row[a.lv.anchor[t]], w);
       if (MODE == kFused && ok) {
@@ -468,8 +479,28 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   KernelTimer kt(ctx, res.kernels);
   AdjProbe probe{g->off, g->adj, g->hub_id, g->hub_bits, words};
 
+  // Symmetry breaking (count only): conditions per position; the last
+  // position is then counted per item, since its list lengths would count
+  // the images the conditions exclude.
+  LevelSym sym[CSM_MAX_QV] = {};
+  res.automorphisms = 1;
+  const bool symmetry = o.symmetry && !want_list;
+  if (symmetry) {
+    std::vector<std::pair<uint32_t, uint32_t>> less;
+    uint64_t aut = 1;
+    if (query_symmetry(q->plan, aut, less) == CSM_OK) {
+      uint32_t pos[CSM_MAX_QV];
+      for (uint32_t p = 0; p < nq; ++p) pos[lv[p].u] = p;
+      for (const auto &c : less) {  // f(v) < f(w): checked where the later of the two is matched
+        const uint32_t pv = pos[c.first], pw = pos[c.second];
+        if (pw > pv) sym[pw].gt[sym[pw].ngt++] = pv;
+        else sym[pv].lt[sym[pv].nlt++] = pw;
+      }
+      res.automorphisms = aut;
+    }
+  }
   // How the last position is counted.
-  const bool precount = !want_list && lv[L].nback == 0;
+  const bool precount = !want_list && lv[L].nback == 0 && !symmetry;
   const bool fused = precount && L >= 2;     // counted from level L-1 items
   const bool rows2 = precount && L == 1;     // two-vertex query: count level-1 rows
   const uint32_t stop = fused ? L - 1 : L;   // deepest level whose items are expanded
@@ -671,6 +702,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     ea.row_b = b;
     ea.k = k;
     ea.lv = lj[k];
+    ea.sym = sym[k];
     ea.mask = ws.mask;
     ea.tile_cnt = ws.tiles;
     ea.tile_off = ws.tiles;

@@ -102,6 +102,20 @@ csm_status csm_plan_build(const csm_host_graph *query, csm_query_plan *out) {
   return csm::build_plan(*query, *out, &g_host_error);
 }
 
+csm_status csm_query_symmetry(const csm_query_plan *plan, uint64_t *automorphisms, uint32_t *num_pairs,
+                              uint32_t *pairs) {
+  if (!plan || !automorphisms || !num_pairs) return CSM_E_ARG;
+  std::vector<std::pair<uint32_t, uint32_t>> less;
+  const csm_status st = csm::query_symmetry(*plan, *automorphisms, less);
+  *num_pairs = static_cast<uint32_t>(less.size());
+  if (pairs)
+    for (size_t i = 0; i < less.size(); ++i) {
+      pairs[2 * i] = less[i].first;
+      pairs[2 * i + 1] = less[i].second;
+    }
+  return st;
+}
+
 csm_status csm_gen_erdos_renyi(uint32_t num_v, uint64_t num_e, uint32_t num_labels, double zipf_s, uint64_t seed,
                                csm_host_graph **out) {
   return make_graph(out, [&](csm_host_graph &g) {

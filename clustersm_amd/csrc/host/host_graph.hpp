@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "csm.h"
@@ -53,5 +54,11 @@ struct QueryCsr {
 csm_status make_query_csr(const csm_host_graph &g, QueryCsr &q);
 
 csm_status build_plan(const csm_host_graph &g, csm_query_plan &plan, std::string *err);
+
+// Symmetry-breaking conditions of the labelled query: pairs (v, w) meaning
+// f(v) < f(w), and |Aut(Q)|.  CSM_E_LIMIT (with no pairs and 1) when the
+// automorphism search needs more than step_budget steps.
+csm_status query_symmetry(const csm_query_plan &plan, uint64_t &automorphisms,
+                          std::vector<std::pair<uint32_t, uint32_t>> &less, uint64_t step_budget = 1u << 22);
 
 }  // namespace csm

@@ -4,6 +4,7 @@
 // order.cpp, main.cu:144) with the same tie-breaks and visiting orders, so
 // that the root and the edge order are identical.
 #include <algorithm>
+#include <memory>
 #include <array>
 #include <deque>
 #include <functional>
@@ -311,4 +312,125 @@ csm_status build_plan(const csm_host_graph &g, csm_query_plan &plan, std::string
   return CSM_OK;
 }
 
+// ------------------------------------------------------------- symmetry
+// Symmetry breaking for counting (no reference counterpart; the reference
+// counts every automorphic image separately).  The labelled automorphism
+// group Aut(Q) is walked as a stabiliser chain: for each vertex v in turn,
+// its orbit under the automorphisms fixing the vertices taken before it is
+// found by searching for one automorphism v -> w per candidate w; then
+// f(v) < f(w) for every other w of the orbit, and v is fixed.  These are the
+// symmetry-breaking conditions of Grochow and Kellis (RECOMB 2007): exactly
+// one embedding of every Aut(Q) class satisfies them, and |Aut(Q)| is the
+// product of the orbit sizes (orbit-stabiliser theorem).
+
+namespace {
+
+struct AutSearch {
+  uint32_t n = 0;
+  const csm_query_plan *plan = nullptr;
+  bool adj[CSM_MAX_QV][CSM_MAX_QV] = {};
+  uint32_t order[CSM_MAX_QV] = {};   // vertices in assignment order
+  uint32_t img[CSM_MAX_QV] = {};     // image of each vertex (kNone: unassigned)
+  bool used[CSM_MAX_QV] = {};
+  uint64_t steps = 0, budget = 0;
+  static constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+  bool fits(uint32_t x, uint32_t y) const {
+    if (plan->labels[x] != plan->labels[y] || plan->degree[x] != plan->degree[y] || used[y]) return false;
+    for (uint32_t z = 0; z < n; ++z)
+      if (img[z] != kNone && adj[x][z] != adj[y][img[z]]) return false;
+    return !(adj[x][x]);
+  }
+  // extends the assignment along order[i..]; false when none exists (or the budget is spent)
+  bool extend(uint32_t i) {
+    if (++steps > budget) return false;
+    if (i == n) return true;
+    const uint32_t x = order[i];
+    if (img[x] != kNone) return extend(i + 1);
+    for (uint32_t y = 0; y < n; ++y) {
+      if (!fits(x, y)) continue;
+      img[x] = y;
+      used[y] = true;
+      if (extend(i + 1)) return true;
+      img[x] = kNone;
+      used[y] = false;
+      if (steps > budget) return false;
+    }
+    return false;
+  }
+  // an automorphism fixing `fixed` pointwise with v -> w
+  bool exists(const std::vector<uint32_t> &fixed, uint32_t v, uint32_t w) {
+    for (uint32_t z = 0; z < n; ++z) {
+      img[z] = kNone;
+      used[z] = false;
+    }
+    for (uint32_t f : fixed) {
+      if (!fits(f, f)) return false;
+      img[f] = f;
+      used[f] = true;
+    }
+    if (img[v] != kNone) return img[v] == w;
+    if (!fits(v, w)) return false;
+    img[v] = w;
+    used[w] = true;
+    return extend(0);
+  }
+};
+
+}  // namespace
+
+csm_status query_symmetry(const csm_query_plan &plan, uint64_t &automorphisms,
+                          std::vector<std::pair<uint32_t, uint32_t>> &less, uint64_t step_budget) {
+  automorphisms = 1;
+  less.clear();
+  const uint32_t n = plan.num_v;
+  if (n < 2 || n > CSM_MAX_QV) return CSM_E_QUERY;
+  std::unique_ptr<AutSearch> S(new AutSearch());
+  S->n = n;
+  S->plan = &plan;
+  S->budget = step_budget;
+  for (uint32_t e = 0; e < plan.num_e; ++e) {
+    const uint32_t a = plan.edge_src[e], b = plan.edge_dst[e];
+    if (a >= n || b >= n) return CSM_E_QUERY;
+    S->adj[a][b] = S->adj[b][a] = true;
+  }
+  // assignment order: breadth first from vertex 0 over the query edges, so
+  // that every vertex after the first has an assigned neighbour to check
+  std::vector<bool> seen(n, false);
+  uint32_t k = 0;
+  for (uint32_t r = 0; r < n; ++r) {
+    if (seen[r]) continue;
+    seen[r] = true;
+    S->order[k++] = r;
+    for (uint32_t h = k - 1; h < k; ++h)
+      for (uint32_t y = 0; y < n; ++y)
+        if (S->adj[S->order[h]][y] && !seen[y]) {
+          seen[y] = true;
+          S->order[k++] = y;
+        }
+  }
+  std::vector<uint32_t> fixed;
+  for (uint32_t v = 0; v < n; ++v) {
+    uint64_t orbit = 1;
+    for (uint32_t w = 0; w < n; ++w) {
+      if (w == v || plan.labels[w] != plan.labels[v] || plan.degree[w] != plan.degree[v]) continue;
+      if (std::find(fixed.begin(), fixed.end(), w) != fixed.end()) continue;
+      const bool found = S->exists(fixed, v, w);
+      if (S->steps > S->budget) {
+        automorphisms = 1;
+        less.clear();
+        return CSM_E_LIMIT;  // too costly to decide: the caller counts without symmetry breaking
+      }
+      if (found) {
+        less.push_back({v, w});
+        ++orbit;
+      }
+    }
+    automorphisms *= orbit;
+    fixed.push_back(v);
+  }
+  return CSM_OK;
+}
+
 }  // namespace csm
+

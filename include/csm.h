@@ -192,6 +192,10 @@ typedef struct csm_match_opts {
   uint64_t item_budget;    /* 0 = none; else the search stops once about this many items (row x candidate
                               tests) are examined: the result's status is CSM_E_BUDGET with the count so
                               far (the call itself returns CSM_OK) */
+  uint32_t symmetry;       /* 1 (count only): count one embedding per Aut(Q) class, as csm_query_symmetry's
+                              conditions select; the result's automorphisms holds |Aut(Q)|, so the full
+                              count is count * automorphisms (when not capped) */
+  uint32_t reserved;
 } csm_match_opts;
 
 /* Kernel classes for per-kernel accounting. */
@@ -230,6 +234,8 @@ typedef struct csm_result {
                               this call (never expected; any makes the call recount its unfinished
                               queries on the plain path and is also reported on stderr) */
   csm_kernel_stat kernels[CSM_K_NUM];
+  uint64_t automorphisms;  /* with the symmetry option: |Aut(Q)| (1 when the search was too costly and
+                              every embedding was counted); otherwise 1 */
 } csm_result;
 
 /* csm_result.path */
@@ -278,6 +284,18 @@ csm_status csm_match_noalloc(csm_ctx *ctx, const csm_graph *g, csm_query *q,
  * unchanged; a capped list then holds the first embeddings of the new order,
  * which is why the default order stays the reference's. */
 csm_status csm_query_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine);
+
+/* Symmetry-breaking conditions for counting, no counterpart in the reference
+ * (which counts every automorphic image; SURVEY's next-step list names it,
+ * behind an option): the labelled automorphism group Aut(Q) walked as a
+ * stabiliser chain, each vertex's orbit under the automorphisms fixing the
+ * vertices before it giving f(v) < f(w) for the orbit's other members
+ * (Grochow & Kellis, RECOMB 2007).  Exactly one embedding of every Aut(Q)
+ * class satisfies the conditions.  pairs (may be NULL): room for
+ * 2 * CSM_MAX_QV * CSM_MAX_QV ids, (v, w) per condition.  CSM_E_LIMIT (with
+ * *automorphisms = 1 and no pairs) when the search is too costly. */
+csm_status csm_query_symmetry(const csm_query_plan *plan, uint64_t *automorphisms, uint32_t *num_pairs,
+                              uint32_t *pairs);
 
 /* ----------------------------------------------------------------- batched
  * No reference counterpart (the reference runs one process per query,

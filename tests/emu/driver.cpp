@@ -1,7 +1,8 @@
 // Runs the library, built for the CPU emulation, on graph files:
 //   driver <data.graph> <cap> <query.graph>...
 // prints per query "plain <count> <capped>", the same from csm_match_noalloc
-// ("noalloc ..."), then its list under a cap of 64
+// ("noalloc ..."), without a cap the count with symmetry breaking times
+// |Aut(Q)| ("plains ..."), then its list under a cap of 64
 // ("list <rows>" and one "row ..." line per embedding, sorted, query-id
 // order), "batch <count> <capped>", and with the candidate-aware order
 // (reorder option) "plainr ..." and "batchr ...", then through the server
@@ -49,6 +50,12 @@ int main(int argc, char **argv) {
     std::printf("plain %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
     check(csm_match_noalloc(ctx, g, q, &o, &r), "match noalloc", ctx);
     std::printf("noalloc %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+    if (cap == 0) {  // symmetry breaking: one embedding per automorphism class
+      csm_match_opts so = o;
+      so.symmetry = 1;
+      check(csm_match(ctx, g, q, &so, &r), "match symmetry", ctx);
+      std::printf("plains %llu %u\n", static_cast<unsigned long long>(r.count * r.automorphisms), r.capped);
+    }
     const uint32_t nq = plans[i].num_v;
     std::vector<uint32_t> rows(64ull * nq);
     csm_match_opts lo{};

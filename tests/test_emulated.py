@@ -57,7 +57,7 @@ def _oracle_lists(dl, de, queries, cap=64):
 
 def _parse(stdout):
     """Per-kind results and the listed rows of each query, from the driver."""
-    got = {"plain": [], "noalloc": [], "batch": [], "plainr": [], "batchr": [], "served": []}
+    got = {"plain": [], "noalloc": [], "plains": [], "batch": [], "plainr": [], "batchr": [], "served": []}
     lists = []
     for ln in stdout.split("\n"):
         f = ln.split()
@@ -94,7 +94,7 @@ def test_device_code_on_cpu_matches_oracle(driver, tmp_path, seed):
         assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
         got, lists = _parse(out.stdout)
         want = _oracle(dl, de, queries, cap)
-        for kind in ("plain", "noalloc", "batch", "plainr", "batchr", "served"):
+        for kind in ("plain", "noalloc", "batch", "plainr", "batchr", "served") + (("plains",) if cap == 0 else ()):
             assert got[kind] == want, (cap, kind, got[kind], want)
         assert lists == _oracle_lists(dl, de, queries), cap
 
@@ -141,6 +141,6 @@ def test_heavy_candidate_lists_on_cpu(driver, tmp_path):
         assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
         got, lists = _parse(out.stdout)
         want = _oracle(dl, de, queries, cap)
-        for kind in ("plain", "noalloc", "batch", "plainr", "batchr", "served"):
+        for kind in ("plain", "noalloc", "batch", "plainr", "batchr", "served") + (("plains",) if cap == 0 else ()):
             assert got[kind] == want, (cap, kind, got[kind], want)
         assert lists == _oracle_lists(dl, de, queries), cap

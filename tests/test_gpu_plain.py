@@ -327,3 +327,32 @@ def test_reorder_keeps_counts_vs_oracle(ctx):
         a.free()
         b.free()
     g.free()
+
+
+def test_symmetry_breaking_counts_vs_oracle(ctx):
+    """With the symmetry option, count x |Aut(Q)| equals the oracle's full
+    count: random graphs and queries (cyclic, few labels, so that many
+    queries have automorphisms), the worked example and the golden cases."""
+    r = random.Random(77)
+    symmetric = 0
+    cases = []
+    for it in range(30):
+        n = r.randint(30, 150)
+        nl = r.randint(1, 2)
+        dl, de = rand_graph(r, n, r.uniform(0.04, 0.2), nl)
+        cases.append((dl, de) + rand_query(r, r.randint(3, 6), r.randint(0, 4), nl))
+    for c in load_golden()[:20]:
+        cases.append((c["data_labels"], c["data_edges"], c["query_labels"], c["query_edges"]))
+    for dl, de, ql, qe in cases:
+        oc, ocap, _, _, _ = O.match(O.Data.from_arrays(dl, de), O.Query.from_arrays(ql, qe),
+                                    O.Query.from_arrays(ql, qe).plan(), threads=4)
+        g = ctx.upload_edges(dl, de)
+        q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+        res, _ = q.match(symmetry=True)
+        assert res.status == 0 and not res.capped
+        assert res.count * res.automorphisms == oc, (ql, qe, res.count, res.automorphisms, oc)
+        symmetric += res.automorphisms > 1 and oc > 0
+        q.free()
+        g.free()
+    print("symmetric queries with embeddings:", symmetric, flush=True)
+    assert symmetric >= 5
