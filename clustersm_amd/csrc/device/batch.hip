@@ -1959,52 +1959,56 @@ using namespace csm;
 
 extern "C" csm_status csm_batch_match(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
                                       const csm_batch_opts *opts, csm_result *out) {
-  if (!ctx || !g || (n && (!qs || !out))) return CSM_E_ARG;
-  csm_batch_opts o{};
-  o.use_graph = 1;  // refinement off by default: the candidate lists already prune dead ends
-  if (opts) o = *opts;
-  std::vector<csm_query *> v(qs, qs + n);
-  for (auto *q : v)
-    if (!q || q->graph_n != g->n) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  return run_batch(ctx, g, v, o, out);
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || (n && (!qs || !out))) return CSM_E_ARG;
+    csm_batch_opts o{};
+    o.use_graph = 1;  // refinement off by default: the candidate lists already prune dead ends
+    if (opts) o = *opts;
+    std::vector<csm_query *> v(qs, qs + n);
+    for (auto *q : v)
+      if (!q || q->graph_n != g->n) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    return run_batch(ctx, g, v, o, out);
+  });
 }
 
 extern "C" csm_status csm_batch_autotune(csm_ctx *ctx, const csm_graph *g, csm_query *const *qs, uint32_t n,
                                          const csm_batch_opts *base, uint32_t sample, csm_batch_opts *out) {
-  if (!ctx || !g || !qs || !out || n == 0) return CSM_E_ARG;
-  csm_batch_opts b{};
-  b.use_graph = 1;
-  if (base) b = *base;
-  for (uint32_t i = 0; i < n; ++i)
-    if (!qs[i] || qs[i]->graph_n != g->n) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  const uint32_t m = std::max<uint32_t>(1, std::min(sample ? sample : 32u, n));
-  std::vector<csm_query *> v;
-  for (uint32_t i = 0; i < m; ++i) v.push_back(qs[static_cast<uint64_t>(i) * n / m]);
-  std::vector<csm_result> res(m);
-  // (workgroups per CU, output blocks per task, block KiB, items per thread)
-  static const uint32_t grid[][4] = {{4, 1, 128, 4}, {3, 1, 128, 4}, {2, 1, 128, 4}, {4, 2, 128, 4},
-                                     {3, 2, 128, 4}, {4, 1, 64, 4},  {4, 1, 256, 4}, {4, 1, 128, 2}};
-  csm_status st = run_batch(ctx, g, v, b, res.data());  // warm-up: code objects, arena layout
-  if (st != CSM_OK) return st;
-  double best = 1e300;
-  csm_batch_opts pick = b;
-  for (const auto &c : grid) {
-    csm_batch_opts t = b;
-    t.wgs_per_cu = c[0];
-    t.max_out = c[1];
-    t.block_kb = c[2];
-    t.ilp = c[3];
-    const auto t0 = std::chrono::steady_clock::now();
-    st = run_batch(ctx, g, v, t, res.data());
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !qs || !out || n == 0) return CSM_E_ARG;
+    csm_batch_opts b{};
+    b.use_graph = 1;
+    if (base) b = *base;
+    for (uint32_t i = 0; i < n; ++i)
+      if (!qs[i] || qs[i]->graph_n != g->n) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    const uint32_t m = std::max<uint32_t>(1, std::min(sample ? sample : 32u, n));
+    std::vector<csm_query *> v;
+    for (uint32_t i = 0; i < m; ++i) v.push_back(qs[static_cast<uint64_t>(i) * n / m]);
+    std::vector<csm_result> res(m);
+    // (workgroups per CU, output blocks per task, block KiB, items per thread)
+    static const uint32_t grid[][4] = {{4, 1, 128, 4}, {3, 1, 128, 4}, {2, 1, 128, 4}, {4, 2, 128, 4},
+                                       {3, 2, 128, 4}, {4, 1, 64, 4},  {4, 1, 256, 4}, {4, 1, 128, 2}};
+    csm_status st = run_batch(ctx, g, v, b, res.data());  // warm-up: code objects, arena layout
     if (st != CSM_OK) return st;
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (ms < best) {
-      best = ms;
-      pick = t;
+    double best = 1e300;
+    csm_batch_opts pick = b;
+    for (const auto &c : grid) {
+      csm_batch_opts t = b;
+      t.wgs_per_cu = c[0];
+      t.max_out = c[1];
+      t.block_kb = c[2];
+      t.ilp = c[3];
+      const auto t0 = std::chrono::steady_clock::now();
+      st = run_batch(ctx, g, v, t, res.data());
+      if (st != CSM_OK) return st;
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (ms < best) {
+        best = ms;
+        pick = t;
+      }
     }
-  }
-  *out = pick;
-  return CSM_OK;
+    *out = pick;
+    return CSM_OK;
+  });
 }

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <map>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -162,6 +163,36 @@ namespace csm {
 
 inline csm_status set_error(csm_ctx *ctx, csm_status st, const std::string &msg) {
   if (ctx) ctx->last_error = msg;
+  return st;
+}
+
+// Body of an exported entry point: an exception thrown inside (host memory
+// exhausted, a thread that cannot start) becomes a status and a message
+// instead of crossing the C ABI.
+template <class F>
+csm_status guarded(csm_ctx *ctx, F &&body) noexcept {
+  csm_status st = CSM_E_DEVICE;
+  const char *what = "unknown exception";
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    st = CSM_E_NOMEM;
+    what = "host memory exhausted";
+  } catch (const std::length_error &) {
+    st = CSM_E_NOMEM;
+    what = "host memory exhausted (container length)";
+  } catch (const std::exception &e) {
+    try {  // the text lives in the exception object: copy it while it exists
+      set_error(ctx, st, std::string("runtime error: ") + e.what());
+    } catch (...) {
+    }
+    return st;
+  } catch (...) {
+  }
+  try {
+    set_error(ctx, st, what);
+  } catch (...) {
+  }
   return st;
 }
 

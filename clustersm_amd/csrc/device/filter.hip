@@ -400,22 +400,26 @@ using namespace csm;
 extern "C" {
 
 csm_status csm_filter(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine, uint64_t *counts) {
-  if (!ctx || !g || !q || q->graph_n != g->n) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  csm_status st = run_filter(ctx, g, q, refine);
-  if (st != CSM_OK) return st;
-  if (counts)
-    for (uint32_t u = 0; u < q->desc.nq; ++u) counts[u] = q->counts[u];
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !q || q->graph_n != g->n) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    csm_status st = run_filter(ctx, g, q, refine);
+    if (st != CSM_OK) return st;
+    if (counts)
+      for (uint32_t u = 0; u < q->desc.nq; ++u) counts[u] = q->counts[u];
+    return CSM_OK;
+  });
 }
 
 csm_status csm_candidates_download(csm_ctx *ctx, const csm_query *q, uint64_t *bits) {
-  if (!ctx || !q || !bits) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(bits, q->bits, 8ull * q->desc.words * q->desc.nq, hipMemcpyDeviceToHost,
-                                  ctx->stream));
-  CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !q || !bits) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(bits, q->bits, 8ull * q->desc.words * q->desc.nq, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CSM_OK;
+  });
 }
 
 }  // extern "C"

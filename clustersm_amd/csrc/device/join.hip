@@ -942,80 +942,86 @@ csm_status apply_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q) {
 using namespace csm;
 
 extern "C" csm_status csm_query_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine) {
-  if (!ctx || !g || !q || q->graph_n != g->n) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  if (!q->filtered) {
-    const csm_status st = run_filter(ctx, g, q, refine);
-    if (st != CSM_OK) return st;
-  }
-  return apply_reorder(ctx, g, q);
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !q || q->graph_n != g->n) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    if (!q->filtered) {
+      const csm_status st = run_filter(ctx, g, q, refine);
+      if (st != CSM_OK) return st;
+    }
+    return apply_reorder(ctx, g, q);
+  });
 }
 
 extern "C" csm_status csm_match(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts *opts,
                                 csm_result *out) {
-  if (!ctx || !g || !q || !out || q->graph_n != g->n) return CSM_E_ARG;
-  csm_match_opts o{};
-  o.refine = 1;
-  if (opts) o = *opts;
-  if (o.list && o.list_capacity == 0) o.list = nullptr;
-  std::memset(out, 0, sizeof(*out));
-  hipSetDevice(ctx->device);
-  auto t0 = std::chrono::steady_clock::now();
-  if (!(o.reuse_filter && q->filtered)) {
-    csm_status st = run_filter(ctx, g, q, o.refine);
-    if (st != CSM_OK) {
-      out->status = st;
-      return st;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !q || !out || q->graph_n != g->n) return CSM_E_ARG;
+    csm_match_opts o{};
+    o.refine = 1;
+    if (opts) o = *opts;
+    if (o.list && o.list_capacity == 0) o.list = nullptr;
+    std::memset(out, 0, sizeof(*out));
+    hipSetDevice(ctx->device);
+    auto t0 = std::chrono::steady_clock::now();
+    if (!(o.reuse_filter && q->filtered)) {
+      csm_status st = run_filter(ctx, g, q, o.refine);
+      if (st != CSM_OK) {
+        out->status = st;
+        return st;
+      }
     }
-  }
-  if (o.reorder) {
-    const csm_status st = apply_reorder(ctx, g, q);
-    if (st != CSM_OK) {
-      out->status = st;
-      return st;
+    if (o.reorder) {
+      const csm_status st = apply_reorder(ctx, g, q);
+      if (st != CSM_OK) {
+        out->status = st;
+        return st;
+      }
     }
-  }
-  for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
-  out->ms_filter = q->ms_filter;
-  csm_status st = run_join(ctx, g, q, o, *out);
-  if (st != CSM_OK || (out->status != CSM_E_TIMEOUT && out->status != CSM_E_BUDGET)) out->status = st;
-  out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  return st;
+    for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
+    out->ms_filter = q->ms_filter;
+    csm_status st = run_join(ctx, g, q, o, *out);
+    if (st != CSM_OK || (out->status != CSM_E_TIMEOUT && out->status != CSM_E_BUDGET)) out->status = st;
+    out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+  });
 }
 
 // csm_match (count only) with every workspace buffer in the library's static
 // device scratch: no hipMalloc, no arena block, no allocating library call.
 extern "C" csm_status csm_match_noalloc(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_match_opts *opts,
                                         csm_result *out) {
-  if (!ctx || !g || !q || !out || q->graph_n != g->n) return CSM_E_ARG;
-  csm_match_opts o{};
-  o.refine = 1;
-  if (opts) o = *opts;
-  std::memset(out, 0, sizeof(*out));
-  if (o.list && o.list_capacity)
-    return set_error(ctx, CSM_E_ARG, "csm_match_noalloc counts only: the sorted list needs arena memory");
-  o.list = nullptr;
-  hipSetDevice(ctx->device);
-  void *region = scratch_region(ctx->device);
-  if (!region) return set_error(ctx, CSM_E_DEVICE, "static scratch not resolved on this device (csm_open first)");
-  WorkMem mem(ctx, region, kScratchBytes);
-  const bool profiling = ctx->profiling;  // no HIP events: they may allocate
-  ctx->profiling = false;
-  auto t0 = std::chrono::steady_clock::now();
-  csm_status st = CSM_OK;
-  if (!(o.reuse_filter && q->filtered)) st = run_filter(ctx, g, q, o.refine, &mem);
-  if (st == CSM_OK && o.reorder) st = apply_reorder(ctx, g, q);
-  if (st == CSM_OK) {
-    for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
-    out->ms_filter = q->ms_filter;
-    mem.release();  // the filter's scratch is not needed by the join
-    st = run_join(ctx, g, q, o, *out, &mem);
-  }
-  ctx->profiling = profiling;
-  if (st == CSM_E_NOMEM)
-    set_error(ctx, st, "workspace beyond the static scratch of " + std::to_string(kScratchBytes >> 20) +
-                           " MiB (" + ctx->last_error + ")");
-  if (st != CSM_OK || (out->status != CSM_E_TIMEOUT && out->status != CSM_E_BUDGET)) out->status = st;
-  out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  return st;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !q || !out || q->graph_n != g->n) return CSM_E_ARG;
+    csm_match_opts o{};
+    o.refine = 1;
+    if (opts) o = *opts;
+    std::memset(out, 0, sizeof(*out));
+    if (o.list && o.list_capacity)
+      return set_error(ctx, CSM_E_ARG, "csm_match_noalloc counts only: the sorted list needs arena memory");
+    o.list = nullptr;
+    hipSetDevice(ctx->device);
+    void *region = scratch_region(ctx->device);
+    if (!region) return set_error(ctx, CSM_E_DEVICE, "static scratch not resolved on this device (csm_open first)");
+    WorkMem mem(ctx, region, kScratchBytes);
+    const bool profiling = ctx->profiling;  // no HIP events: they may allocate
+    ctx->profiling = false;
+    auto t0 = std::chrono::steady_clock::now();
+    csm_status st = CSM_OK;
+    if (!(o.reuse_filter && q->filtered)) st = run_filter(ctx, g, q, o.refine, &mem);
+    if (st == CSM_OK && o.reorder) st = apply_reorder(ctx, g, q);
+    if (st == CSM_OK) {
+      for (int c = 0; c < CSM_K_NUM; ++c) out->kernels[c] = q->kstat[c];
+      out->ms_filter = q->ms_filter;
+      mem.release();  // the filter's scratch is not needed by the join
+      st = run_join(ctx, g, q, o, *out, &mem);
+    }
+    ctx->profiling = profiling;
+    if (st == CSM_E_NOMEM)
+      set_error(ctx, st, "workspace beyond the static scratch of " + std::to_string(kScratchBytes >> 20) +
+                             " MiB (" + ctx->last_error + ")");
+    if (st != CSM_OK || (out->status != CSM_E_TIMEOUT && out->status != CSM_E_BUDGET)) out->status = st;
+    out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+  });
 }

@@ -308,7 +308,8 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
   hipStream_t s = ctx->stream;
   uint32_t shift = 1;
   while ((1ull << shift) <= nv) ++shift;  // 2^shift > nv: the sentinel is above every key
-  if (2 * shift > 64) return set_error(ctx, CSM_E_LIMIT, "vertex ids too wide");
+  // keys are (src << shift) | dst below a sentinel of 2^(2 shift) - 1
+  if (2 * shift >= 64) return set_error(ctx, CSM_E_LIMIT, "vertex ids too wide (2^31 vertices or more)");
   const uint64_t nk = 2 * m;
   std::unique_ptr<csm_graph> g(new csm_graph());
   g->n = nv;
@@ -319,35 +320,10 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
     ctx->arena.free(g->off);
     return set_error(ctx, CSM_E_NOMEM, "arena exhausted (graph arrays)");
   }
-  auto fail = [&](csm_status st, const std::string &msg) {
-    ctx->arena.free(g->label);
-    ctx->arena.free(g->off);
-    ctx->arena.free(g->adj);
-    ctx->arena.free(g->hub_id);
-    ctx->arena.free(g->hub_bits);
-    return set_error(ctx, st, msg);
-  };
-  if (nv) CSM_HIP_TRY(ctx, hipMemcpyAsync(g->label, labels, 4ull * nv, hipMemcpyHostToDevice, s));
-  uint32_t mx = 0;
-  for (uint32_t v = 0; v < nv; ++v) mx = std::max(mx, labels[v]);
-  g->max_label = mx;
-  if (m == 0) {
-    fill_off_empty<<<grid_for(nv + 1ull), 256, 0, s>>>(g->off, nv);
-    g->adj = static_cast<uint32_t *>(ctx->arena.alloc(4));
-    CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
-    csm_status hst = build_hubs(ctx, g.get());
-    if (hst == CSM_OK) hst = build_nlc(ctx, g.get());
-    if (hst != CSM_OK) return hst;
-    *out = g.release();
-    return CSM_OK;
-  }
-  // temporaries: edges, keys x2, flags/positions, partials, sort scratch
-  uint32_t *d_edges = static_cast<uint32_t *>(ctx->arena.alloc(8 * m));
-  uint64_t *keys = static_cast<uint64_t *>(ctx->arena.alloc(8 * nk));
-  uint64_t *keys2 = static_cast<uint64_t *>(ctx->arena.alloc(8 * nk));
-  uint64_t *pos = static_cast<uint64_t *>(ctx->arena.alloc(8 * nk));
-  uint64_t *partials = static_cast<uint64_t *>(ctx->arena.alloc(8 * (nk / kScanTile + 2)));
-  uint64_t *total_d = static_cast<uint64_t *>(ctx->arena.alloc(64));
+  // temporaries of the CSR build (freed on every exit) and the graph's own
+  // arrays (freed on a failure)
+  uint32_t *d_edges = nullptr;
+  uint64_t *keys = nullptr, *keys2 = nullptr, *pos = nullptr, *partials = nullptr, *total_d = nullptr;
   auto free_tmp = [&]() {
     ctx->arena.free(d_edges);
     ctx->arena.free(keys);
@@ -355,12 +331,52 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
     ctx->arena.free(pos);
     ctx->arena.free(partials);
     ctx->arena.free(total_d);
+    d_edges = nullptr;
+    keys = keys2 = pos = partials = total_d = nullptr;
   };
+  auto fail = [&](csm_status st, const std::string &msg) {
+    hipStreamSynchronize(s);  // nothing in flight uses what is freed
+    free_tmp();
+    ctx->arena.free(g->label);
+    ctx->arena.free(g->off);
+    ctx->arena.free(g->adj);
+    ctx->arena.free(g->hub_id);
+    ctx->arena.free(g->hub_bits);
+    ctx->arena.free(g->nlc);
+    return set_error(ctx, st, msg);
+  };
+#define UPLOAD_TRY(expr)                                                             \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) return fail(CSM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+  if (nv) UPLOAD_TRY(hipMemcpyAsync(g->label, labels, 4ull * nv, hipMemcpyHostToDevice, s));
+  uint32_t mx = 0;
+  for (uint32_t v = 0; v < nv; ++v) mx = std::max(mx, labels[v]);
+  g->max_label = mx;
+  if (m == 0) {
+    fill_off_empty<<<grid_for(nv + 1ull), 256, 0, s>>>(g->off, nv);
+    g->adj = static_cast<uint32_t *>(ctx->arena.alloc(4));
+    if (!g->adj) return fail(CSM_E_NOMEM, "arena exhausted (adjacency)");
+    UPLOAD_TRY(hipStreamSynchronize(s));
+    csm_status hst = build_hubs(ctx, g.get());
+    if (hst == CSM_OK) hst = build_nlc(ctx, g.get());
+    if (hst != CSM_OK) return fail(hst, std::string(ctx->last_error));
+    *out = g.release();
+    return CSM_OK;
+  }
+  // temporaries: edges, keys x2, flags/positions, partials, sort scratch
+  d_edges = static_cast<uint32_t *>(ctx->arena.alloc(8 * m));
+  keys = static_cast<uint64_t *>(ctx->arena.alloc(8 * nk));
+  keys2 = static_cast<uint64_t *>(ctx->arena.alloc(8 * nk));
+  pos = static_cast<uint64_t *>(ctx->arena.alloc(8 * nk));
+  partials = static_cast<uint64_t *>(ctx->arena.alloc(8 * (nk / kScanTile + 2)));
+  total_d = static_cast<uint64_t *>(ctx->arena.alloc(64));
   if (!d_edges || !keys || !keys2 || !pos || !partials || !total_d) {
     free_tmp();
     return fail(CSM_E_NOMEM, "arena exhausted (CSR build)");
   }
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(d_edges, edges, 8 * m, hipMemcpyHostToDevice, s));
+  UPLOAD_TRY(hipMemcpyAsync(d_edges, edges, 8 * m, hipMemcpyHostToDevice, s));
   make_keys<<<grid_for(m), 256, 0, s>>>(d_edges, m, shift, keys);
   rocprim::double_buffer<uint64_t> db(keys, keys2);
   size_t tmp_bytes = 0;
@@ -380,8 +396,8 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
   mark_unique<<<grid_for(nk), 256, 0, s>>>(sorted, nk, shift, pos);
   scan_u64_exclusive(pos, nk, partials, total_d, s);
   uint64_t total = 0;
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(&total, total_d, 8, hipMemcpyDeviceToHost, s));
-  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  UPLOAD_TRY(hipMemcpyAsync(&total, total_d, 8, hipMemcpyDeviceToHost, s));
+  UPLOAD_TRY(hipStreamSynchronize(s));
   if (total >= 0xFFFFFFFFull) {
     free_tmp();
     return fail(CSM_E_LIMIT, "2|E| does not fit the uint32 CSR");
@@ -394,19 +410,20 @@ csm_status upload_graph_device(csm_ctx *ctx, uint32_t nv, const uint32_t *labels
   if (total == 0) fill_off_empty<<<grid_for(nv + 1ull), 256, 0, s>>>(g->off, nv);
   else scatter_csr<<<grid_for(nk), 256, 0, s>>>(sorted, nk, shift, pos, total, nv, g->off, g->adj);
   uint32_t *d_max = reinterpret_cast<uint32_t *>(total_d);
-  CSM_HIP_TRY(ctx, hipMemsetAsync(d_max, 0, 4, s));
+  UPLOAD_TRY(hipMemsetAsync(d_max, 0, 4, s));
   max_degree_kernel<<<grid_for(nv), 256, 0, s>>>(g->off, nv, d_max);
   uint32_t maxdeg = 0;
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(&maxdeg, d_max, 4, hipMemcpyDeviceToHost, s));
-  CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+  UPLOAD_TRY(hipMemcpyAsync(&maxdeg, d_max, 4, hipMemcpyDeviceToHost, s));
+  UPLOAD_TRY(hipStreamSynchronize(s));
   free_tmp();
   g->m = total / 2;
   g->max_degree = maxdeg;
   csm_status hst = build_hubs(ctx, g.get());
   if (hst == CSM_OK) hst = build_nlc(ctx, g.get());
-  if (hst != CSM_OK) return fail(hst, ctx->last_error);
+  if (hst != CSM_OK) return fail(hst, std::string(ctx->last_error));
   *out = g.release();
   return CSM_OK;
+#undef UPLOAD_TRY
 }
 
 // ------------------------------------------------------------ query desc
@@ -475,50 +492,54 @@ using namespace csm;
 extern "C" {
 
 csm_status csm_open(int device, uint64_t arena_bytes, csm_ctx **out) {
-  if (!out) return CSM_E_ARG;
-  *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return CSM_E_NO_GPU;
-  if (device < 0 || device >= count) return CSM_E_ARG;
-  if (hipSetDevice(device) != hipSuccess) return CSM_E_DEVICE;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CSM_E_DEVICE;
-  std::unique_ptr<csm_ctx> ctx(new csm_ctx());
-  ctx->device = device;
-  ctx->num_cus = static_cast<uint32_t>(prop.multiProcessorCount);
-  ctx->arch = prop.gcnArchName;
-  if (ctx->arch.rfind("gfx950", 0) != 0) return CSM_E_NO_GPU;  // built for gfx950 only
-  if (scratch_setup(device) != CSM_OK) return CSM_E_DEVICE;  // device current
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return CSM_E_DEVICE;
-  if (hipHostMalloc(reinterpret_cast<void **>(&ctx->pinned), 4096, hipHostMallocDefault) != hipSuccess) {
-    hipStreamDestroy(ctx->stream);
-    return CSM_E_DEVICE;
-  }
-  size_t free_b = 0, total_b = 0;
-  hipMemGetInfo(&free_b, &total_b);
-  if (arena_bytes == 0) arena_bytes = static_cast<uint64_t>(free_b * 0.85);
-  if (arena_bytes > free_b) arena_bytes = free_b - (free_b >> 6);
-  csm_status st = ctx->arena.init(arena_bytes);
-  if (st != CSM_OK) {
-    hipHostFree(ctx->pinned);
-    hipStreamDestroy(ctx->stream);
-    return st;
-  }
-  *out = ctx.release();
-  return CSM_OK;
+  return csm::guarded(nullptr, [&]() -> csm_status {
+    if (!out) return CSM_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return CSM_E_NO_GPU;
+    if (device < 0 || device >= count) return CSM_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CSM_E_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CSM_E_DEVICE;
+    std::unique_ptr<csm_ctx> ctx(new csm_ctx());
+    ctx->device = device;
+    ctx->num_cus = static_cast<uint32_t>(prop.multiProcessorCount);
+    ctx->arch = prop.gcnArchName;
+    if (ctx->arch.rfind("gfx950", 0) != 0) return CSM_E_NO_GPU;  // built for gfx950 only
+    if (scratch_setup(device) != CSM_OK) return CSM_E_DEVICE;  // device current
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return CSM_E_DEVICE;
+    if (hipHostMalloc(reinterpret_cast<void **>(&ctx->pinned), 4096, hipHostMallocDefault) != hipSuccess) {
+      hipStreamDestroy(ctx->stream);
+      return CSM_E_DEVICE;
+    }
+    size_t free_b = 0, total_b = 0;
+    hipMemGetInfo(&free_b, &total_b);
+    if (arena_bytes == 0) arena_bytes = static_cast<uint64_t>(free_b * 0.85);
+    if (arena_bytes > free_b) arena_bytes = free_b - (free_b >> 6);
+    csm_status st = ctx->arena.init(arena_bytes);
+    if (st != CSM_OK) {
+      hipHostFree(ctx->pinned);
+      hipStreamDestroy(ctx->stream);
+      return st;
+    }
+    *out = ctx.release();
+    return CSM_OK;
+  });
 }
 
 csm_status csm_close(csm_ctx *ctx) {
-  if (!ctx) return CSM_E_ARG;
-  csm_serve_stop(ctx);
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
-  ctx->arena.release();
-  if (ctx->pinned) hipHostFree(ctx->pinned);
-  hipStreamDestroy(ctx->stream);
-  delete ctx;
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx) return CSM_E_ARG;
+    csm_serve_stop(ctx);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
+    ctx->arena.release();
+    if (ctx->pinned) hipHostFree(ctx->pinned);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CSM_OK;
+  });
 }
 
 const char *csm_last_error(const csm_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
@@ -530,35 +551,37 @@ csm_status csm_set_profiling(csm_ctx *ctx, int on) {
 }
 
 csm_status csm_copy_probe(csm_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms_per_copy) {
-  if (!ctx || !ms_per_copy || bytes < 16 || iters == 0) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  bytes &= ~15ull;
-  void *src = ctx->arena.alloc(bytes), *dst = ctx->arena.alloc(bytes);
-  if (!src || !dst) {
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !ms_per_copy || bytes < 16 || iters == 0) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    bytes &= ~15ull;
+    void *src = ctx->arena.alloc(bytes), *dst = ctx->arena.alloc(bytes);
+    if (!src || !dst) {
+      ctx->arena.free(src);
+      ctx->arena.free(dst);
+      return set_error(ctx, CSM_E_NOMEM, "arena exhausted (copy probe)");
+    }
+    const uint64_t n = bytes / 16;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(ctx->num_cus * 8ull, (n + 255) / 256));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    copy_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);  // warm
+    hipEventRecord(e0, ctx->stream);
+    for (uint32_t i = 0; i < iters; ++i)
+      copy_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+    hipEventRecord(e1, ctx->stream);
+    const hipError_t err = hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
     ctx->arena.free(src);
     ctx->arena.free(dst);
-    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (copy probe)");
-  }
-  const uint64_t n = bytes / 16;
-  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(ctx->num_cus * 8ull, (n + 255) / 256));
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  copy_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);  // warm
-  hipEventRecord(e0, ctx->stream);
-  for (uint32_t i = 0; i < iters; ++i)
-    copy_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
-  hipEventRecord(e1, ctx->stream);
-  const hipError_t err = hipEventSynchronize(e1);
-  float ms = 0;
-  hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  ctx->arena.free(src);
-  ctx->arena.free(dst);
-  if (err != hipSuccess) return set_error(ctx, CSM_E_DEVICE, hipGetErrorString(err));
-  *ms_per_copy = ms / iters;
-  return CSM_OK;
+    if (err != hipSuccess) return set_error(ctx, CSM_E_DEVICE, hipGetErrorString(err));
+    *ms_per_copy = ms / iters;
+    return CSM_OK;
+  });
 }
 
 csm_status csm_ctx_query(const csm_ctx *ctx, csm_ctx_info *out) {
@@ -574,18 +597,22 @@ csm_status csm_ctx_query(const csm_ctx *ctx, csm_ctx_info *out) {
 }
 
 csm_status csm_graph_upload(csm_ctx *ctx, const csm_host_graph *g, csm_graph **out) {
-  if (!ctx || !g || !out) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  return upload_graph_device(ctx, g->n, g->labels.data(), g->m, g->edges.data(), out);
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !out) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    return upload_graph_device(ctx, g->n, g->labels.data(), g->m, g->edges.data(), out);
+  });
 }
 
 csm_status csm_graph_upload_edges(csm_ctx *ctx, uint32_t num_v, const uint32_t *labels, uint64_t num_e,
                                   const uint32_t *edges, csm_graph **out) {
-  if (!ctx || !out || (num_v && !labels) || (num_e && !edges)) return CSM_E_ARG;
-  for (uint64_t i = 0; i < 2 * num_e; ++i)
-    if (edges[i] >= num_v) return set_error(ctx, CSM_E_ARG, "edge endpoint out of range");
-  hipSetDevice(ctx->device);
-  return upload_graph_device(ctx, num_v, labels, num_e, edges, out);
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !out || (num_v && !labels) || (num_e && !edges)) return CSM_E_ARG;
+    for (uint64_t i = 0; i < 2 * num_e; ++i)
+      if (edges[i] >= num_v) return set_error(ctx, CSM_E_ARG, "edge endpoint out of range");
+    hipSetDevice(ctx->device);
+    return upload_graph_device(ctx, num_v, labels, num_e, edges, out);
+  });
 }
 
 csm_status csm_graph_free(csm_ctx *ctx, csm_graph *g) {
@@ -611,66 +638,72 @@ csm_status csm_graph_info(const csm_graph *g, uint32_t *num_v, uint64_t *num_e, 
 
 csm_status csm_graph_download(csm_ctx *ctx, const csm_graph *g, uint32_t *offsets, uint32_t *adj,
                               uint32_t *labels) {
-  if (!ctx || !g) return CSM_E_ARG;
-  hipSetDevice(ctx->device);
-  if (offsets) CSM_HIP_TRY(ctx, hipMemcpyAsync(offsets, g->off, 4ull * (g->n + 1ull), hipMemcpyDeviceToHost, ctx->stream));
-  if (adj && g->m) CSM_HIP_TRY(ctx, hipMemcpyAsync(adj, g->adj, 8ull * g->m, hipMemcpyDeviceToHost, ctx->stream));
-  if (labels && g->n) CSM_HIP_TRY(ctx, hipMemcpyAsync(labels, g->label, 4ull * g->n, hipMemcpyDeviceToHost, ctx->stream));
-  CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g) return CSM_E_ARG;
+    hipSetDevice(ctx->device);
+    if (offsets) CSM_HIP_TRY(ctx, hipMemcpyAsync(offsets, g->off, 4ull * (g->n + 1ull), hipMemcpyDeviceToHost, ctx->stream));
+    if (adj && g->m) CSM_HIP_TRY(ctx, hipMemcpyAsync(adj, g->adj, 8ull * g->m, hipMemcpyDeviceToHost, ctx->stream));
+    if (labels && g->n) CSM_HIP_TRY(ctx, hipMemcpyAsync(labels, g->label, 4ull * g->n, hipMemcpyDeviceToHost, ctx->stream));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CSM_OK;
+  });
 }
 
 csm_status csm_graph_to_host(csm_ctx *ctx, const csm_graph *g, csm_host_graph **out) {
-  if (!ctx || !g || !out) return CSM_E_ARG;
-  std::vector<uint32_t> off(g->n + 1ull), adj(2 * g->m);
-  std::unique_ptr<csm_host_graph> h(new csm_host_graph());
-  h->n = g->n;
-  h->labels.resize(g->n);
-  csm_status st = csm_graph_download(ctx, g, off.data(), adj.data(), h->labels.data());
-  if (st != CSM_OK) return st;
-  h->edges.reserve(2 * g->m);
-  for (uint32_t v = 0; v < g->n; ++v)
-    for (uint32_t o = off[v]; o < off[v + 1]; ++o)
-      if (adj[o] > v) {
-        h->edges.push_back(v);
-        h->edges.push_back(adj[o]);
-      }
-  h->m = h->edges.size() / 2;
-  h->off.assign(off.begin(), off.end());
-  h->adj = std::move(adj);
-  h->csr_ready = true;
-  *out = h.release();
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !out) return CSM_E_ARG;
+    std::vector<uint32_t> off(g->n + 1ull), adj(2 * g->m);
+    std::unique_ptr<csm_host_graph> h(new csm_host_graph());
+    h->n = g->n;
+    h->labels.resize(g->n);
+    csm_status st = csm_graph_download(ctx, g, off.data(), adj.data(), h->labels.data());
+    if (st != CSM_OK) return st;
+    h->edges.reserve(2 * g->m);
+    for (uint32_t v = 0; v < g->n; ++v)
+      for (uint32_t o = off[v]; o < off[v + 1]; ++o)
+        if (adj[o] > v) {
+          h->edges.push_back(v);
+          h->edges.push_back(adj[o]);
+        }
+    h->m = h->edges.size() / 2;
+    h->off.assign(off.begin(), off.end());
+    h->adj = std::move(adj);
+    h->csr_ready = true;
+    *out = h.release();
+    return CSM_OK;
+  });
 }
 
 csm_status csm_query_prepare(csm_ctx *ctx, const csm_graph *g, const csm_query_plan *plan, csm_query **out) {
-  if (!ctx || !g || !plan || !out) return CSM_E_ARG;
-  std::unique_ptr<csm_query> q(new csm_query());
-  q->plan = *plan;
-  csm_status st = query_build_desc(*plan, g->n, q->desc, q->level);
-  if (st != CSM_OK) return set_error(ctx, st, "invalid query plan");
-  q->graph_n = g->n;
-  uint64_t bytes = 8ull * q->desc.words * q->desc.nq;
-  q->bits = static_cast<uint64_t *>(ctx->arena.alloc(bytes));
-  q->d_desc = static_cast<QueryDesc *>(ctx->arena.alloc(sizeof(QueryDesc)));
-  q->d_cluster_us = static_cast<uint32_t *>(ctx->arena.alloc(4ull * std::max<uint32_t>(plan->num_cluster_us, 1)));
-  q->d_cluster_off = static_cast<uint32_t *>(ctx->arena.alloc(4ull * (plan->num_clusters + 1ull)));
-  if (!q->bits || !q->d_desc || !q->d_cluster_us || !q->d_cluster_off) {
-    ctx->arena.free(q->bits);
-    ctx->arena.free(q->d_desc);
-    ctx->arena.free(q->d_cluster_us);
-    ctx->arena.free(q->d_cluster_off);
-    return set_error(ctx, CSM_E_NOMEM, "arena exhausted (query)");
-  }
-  hipSetDevice(ctx->device);
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_desc, &q->desc, sizeof(QueryDesc), hipMemcpyHostToDevice, ctx->stream));
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_cluster_us, plan->cluster_us, 4ull * plan->num_cluster_us,
-                                  hipMemcpyHostToDevice, ctx->stream));
-  CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_cluster_off, plan->cluster_offsets, 4ull * (plan->num_clusters + 1ull),
-                                  hipMemcpyHostToDevice, ctx->stream));
-  CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  *out = q.release();
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g || !plan || !out) return CSM_E_ARG;
+    std::unique_ptr<csm_query> q(new csm_query());
+    q->plan = *plan;
+    csm_status st = query_build_desc(*plan, g->n, q->desc, q->level);
+    if (st != CSM_OK) return set_error(ctx, st, "invalid query plan");
+    q->graph_n = g->n;
+    uint64_t bytes = 8ull * q->desc.words * q->desc.nq;
+    q->bits = static_cast<uint64_t *>(ctx->arena.alloc(bytes));
+    q->d_desc = static_cast<QueryDesc *>(ctx->arena.alloc(sizeof(QueryDesc)));
+    q->d_cluster_us = static_cast<uint32_t *>(ctx->arena.alloc(4ull * std::max<uint32_t>(plan->num_cluster_us, 1)));
+    q->d_cluster_off = static_cast<uint32_t *>(ctx->arena.alloc(4ull * (plan->num_clusters + 1ull)));
+    if (!q->bits || !q->d_desc || !q->d_cluster_us || !q->d_cluster_off) {
+      ctx->arena.free(q->bits);
+      ctx->arena.free(q->d_desc);
+      ctx->arena.free(q->d_cluster_us);
+      ctx->arena.free(q->d_cluster_off);
+      return set_error(ctx, CSM_E_NOMEM, "arena exhausted (query)");
+    }
+    hipSetDevice(ctx->device);
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_desc, &q->desc, sizeof(QueryDesc), hipMemcpyHostToDevice, ctx->stream));
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_cluster_us, plan->cluster_us, 4ull * plan->num_cluster_us,
+                                    hipMemcpyHostToDevice, ctx->stream));
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(q->d_cluster_off, plan->cluster_offsets, 4ull * (plan->num_clusters + 1ull),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    CSM_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *out = q.release();
+    return CSM_OK;
+  });
 }
 
 csm_status csm_query_free(csm_ctx *ctx, csm_query *q) {

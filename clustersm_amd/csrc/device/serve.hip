@@ -383,39 +383,44 @@ using namespace csm;
 extern "C" {
 
 csm_status csm_serve_start(csm_ctx *ctx, const csm_graph *g, const csm_serve_opts *opts) {
-  if (!ctx || !g) return CSM_E_ARG;
-  if (ctx->server) return set_error(ctx, CSM_E_BUSY, "a server is already running on this context");
-  csm_serve_opts o{};
-  if (opts) o = *opts;
-  Server *srv = new Server(ctx, g, o);
-  csm_status st = srv->start();
-  if (st != CSM_OK) {
-    delete srv;
-    return st;
-  }
-  ctx->server = srv;
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !g) return CSM_E_ARG;
+    if (ctx->server) return set_error(ctx, CSM_E_BUSY, "a server is already running on this context");
+    csm_serve_opts o{};
+    if (opts) o = *opts;
+    std::unique_ptr<Server> srv(new Server(ctx, g, o));
+    csm_status st = srv->start();
+    if (st != CSM_OK) return st;
+    ctx->server = srv.release();
+    return CSM_OK;
+  });
 }
 
 csm_status csm_serve_submit(csm_ctx *ctx, const csm_query_plan *plan, uint64_t *ticket) {
-  if (!ctx || !ctx->server) return CSM_E_ARG;
-  return ctx->server->submit(plan, ticket);
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !ctx->server) return CSM_E_ARG;
+    return ctx->server->submit(plan, ticket);
+  });
 }
 
 csm_status csm_serve_poll(csm_ctx *ctx, uint64_t ticket, int wait, csm_result *out) {
-  if (!ctx || !ctx->server) return CSM_E_ARG;
-  return ctx->server->poll(ticket, wait, out);
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx || !ctx->server) return CSM_E_ARG;
+    return ctx->server->poll(ticket, wait, out);
+  });
 }
 
 // Finishes every submitted request, then stops the worker.  Results not yet
 // polled are dropped with the server.
 csm_status csm_serve_stop(csm_ctx *ctx) {
-  if (!ctx) return CSM_E_ARG;
-  if (!ctx->server) return CSM_OK;
-  ctx->server->stop();
-  delete ctx->server;
-  ctx->server = nullptr;
-  return CSM_OK;
+  return csm::guarded(ctx, [&]() -> csm_status {
+    if (!ctx) return CSM_E_ARG;
+    if (!ctx->server) return CSM_OK;
+    ctx->server->stop();
+    delete ctx->server;
+    ctx->server = nullptr;
+    return CSM_OK;
+  });
 }
 
 }  // extern "C"

@@ -1,5 +1,7 @@
 // C ABI of the host layer: graphs, planner, generators, status strings.
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 
 #include "host_graph.hpp"
@@ -18,14 +20,41 @@ csm_status gen_query(csm_host_graph &data, uint32_t nq, uint32_t mode, uint32_t 
 namespace {
 thread_local std::string g_host_error;
 
+// Body of an exported host entry point: an exception thrown inside (host
+// memory exhausted) becomes a status and a message instead of crossing the
+// C ABI.
+template <class F>
+csm_status host_guarded(F &&body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+  } catch (const std::length_error &) {
+  } catch (const std::exception &e) {
+    try {
+      g_host_error = std::string("runtime error: ") + e.what();
+    } catch (...) {
+    }
+    return CSM_E_ARG;
+  } catch (...) {
+    return CSM_E_ARG;
+  }
+  try {
+    g_host_error = "host memory exhausted";
+  } catch (...) {
+  }
+  return CSM_E_NOMEM;
+}
+
 template <class F>
 csm_status make_graph(csm_host_graph **out, F f) {
   if (!out) return CSM_E_ARG;
   *out = nullptr;
-  std::unique_ptr<csm_host_graph> g(new csm_host_graph());
-  csm_status st = f(*g);
-  if (st == CSM_OK) *out = g.release();
-  return st;
+  return host_guarded([&]() -> csm_status {
+    std::unique_ptr<csm_host_graph> g(new csm_host_graph());
+    csm_status st = f(*g);
+    if (st == CSM_OK) *out = g.release();
+    return st;
+  });
 }
 }  // namespace
 
@@ -40,7 +69,7 @@ const char *csm_status_string(csm_status s) {
     case CSM_E_QUERY: return "query outside the supported envelope";
     case CSM_E_PLAN: return "planner limits exceeded";
     case CSM_E_DEVICE: return "HIP runtime error";
-    case CSM_E_NOMEM: return "device arena exhausted";
+    case CSM_E_NOMEM: return "device arena or host memory exhausted";
     case CSM_E_NO_GPU: return "no gfx950 device";
     case CSM_E_BUSY: return "serving queue full";
     case CSM_E_NOT_FOUND: return "unknown ticket";
@@ -84,12 +113,12 @@ csm_status csm_host_graph_views(const csm_host_graph *g, const uint32_t **labels
 
 csm_status csm_host_graph_write(const csm_host_graph *g, const char *path) {
   if (!g || !path) return CSM_E_ARG;
-  return csm::write_graph_file(*g, path);
+  return host_guarded([&]() { return csm::write_graph_file(*g, path); });
 }
 
 csm_status csm_host_graph_write_bin(const csm_host_graph *g, const char *path) {
   if (!g || !path) return CSM_E_ARG;
-  return csm::write_graph_bin(*g, path);
+  return host_guarded([&]() { return csm::write_graph_bin(*g, path); });
 }
 
 csm_status csm_host_graph_read_bin(const char *path, csm_host_graph **out) {
@@ -99,21 +128,23 @@ csm_status csm_host_graph_read_bin(const char *path, csm_host_graph **out) {
 
 csm_status csm_plan_build(const csm_host_graph *query, csm_query_plan *out) {
   if (!query || !out) return CSM_E_ARG;
-  return csm::build_plan(*query, *out, &g_host_error);
+  return host_guarded([&]() { return csm::build_plan(*query, *out, &g_host_error); });
 }
 
 csm_status csm_query_symmetry(const csm_query_plan *plan, uint64_t *automorphisms, uint32_t *num_pairs,
                               uint32_t *pairs) {
   if (!plan || !automorphisms || !num_pairs) return CSM_E_ARG;
-  std::vector<std::pair<uint32_t, uint32_t>> less;
-  const csm_status st = csm::query_symmetry(*plan, *automorphisms, less);
-  *num_pairs = static_cast<uint32_t>(less.size());
-  if (pairs)
-    for (size_t i = 0; i < less.size(); ++i) {
-      pairs[2 * i] = less[i].first;
-      pairs[2 * i + 1] = less[i].second;
-    }
-  return st;
+  return host_guarded([&]() -> csm_status {
+    std::vector<std::pair<uint32_t, uint32_t>> less;
+    const csm_status st = csm::query_symmetry(*plan, *automorphisms, less);
+    *num_pairs = static_cast<uint32_t>(less.size());
+    if (pairs)
+      for (size_t i = 0; i < less.size(); ++i) {
+        pairs[2 * i] = less[i].first;
+        pairs[2 * i + 1] = less[i].second;
+      }
+    return st;
+  });
 }
 
 csm_status csm_gen_erdos_renyi(uint32_t num_v, uint64_t num_e, uint32_t num_labels, double zipf_s, uint64_t seed,

@@ -181,6 +181,17 @@ csm_status read_graph_file(const std::string &path, csm_host_graph &g, std::stri
     if (err) *err = "vertex count exceeds uint32";
     return CSM_E_FORMAT;
   }
+  // A vertex record takes at least 8 bytes ("v 0 0 0" and a separator), an
+  // edge record at least 6 ("e 0 1"); a header that promises more records
+  // than the rest of the file can hold is rejected before anything is sized
+  // by it.
+  const uint64_t rest = static_cast<uint64_t>(buf.data() + buf.size() - r.p);
+  if (n > rest / 7 || m > rest / 5 || 7 * n + 5 * m > rest + 1) {
+    if (err)
+      *err = "header declares " + std::to_string(n) + " vertices and " + std::to_string(m) +
+             " edges, more than the " + std::to_string(rest) + " bytes after it can hold";
+    return CSM_E_FORMAT;
+  }
   g = csm_host_graph();
   g.n = static_cast<uint32_t>(n);
   g.m = m;

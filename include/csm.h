@@ -38,7 +38,7 @@ typedef enum csm_status {
   CSM_E_QUERY = 4,      /* query outside the envelope: |V|<2 or >32, not simple, disconnected */
   CSM_E_PLAN = 5,       /* planner limits exceeded (clusters, compact size) */
   CSM_E_DEVICE = 6,     /* HIP runtime error (cuchk exits with the code, cuda_helpers.h:16-33) */
-  CSM_E_NOMEM = 7,      /* device arena exhausted */
+  CSM_E_NOMEM = 7,      /* device arena exhausted, or host memory (an entry point that cannot allocate on the host) */
   CSM_E_NO_GPU = 8,     /* no usable gfx950 device */
   CSM_E_BUSY = 9,       /* serving admission queue full */
   CSM_E_NOT_FOUND = 10, /* unknown serving ticket */

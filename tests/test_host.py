@@ -91,6 +91,19 @@ def test_parser_errors(tmp_path):
         assert e.value.status == csm.E_FORMAT, text
 
 
+def test_parser_rejects_headers_larger_than_the_file(tmp_path):
+    """A header promising more records than the file can hold is an error
+    before anything is sized by it (no multi-GB allocation for a 30-byte file)."""
+    f = tmp_path / "g.in"
+    for text in ["t 4000000000 1\n", "t 2 900000000000000000\nv 0 0 0\nv 1 0 0\n", "t 3 0\nv 0 0 0\nv 1 0 0\n"]:
+        f.write_text(text)
+        with pytest.raises(csm.CsmError) as e:
+            csm.HostGraph.read(str(f))
+        assert e.value.status == csm.E_FORMAT and "bytes after it can hold" in str(e.value), (text, str(e.value))
+    f.write_text("t 2 1\nv 0 0 1\nv 1 0 1\ne 0 1")  # the smallest records, no final newline
+    assert csm.HostGraph.read(str(f)).arrays()[1].tolist() == [[0, 1]]
+
+
 def test_parser_roundtrip_and_edge_order(tmp_path):
     qp, gp = example_paths()
     g = csm.HostGraph.read(gp)
