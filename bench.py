@@ -604,17 +604,20 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
                     f"queries/s of random-walk queries, 4..32 vertices (geometric, mean 10), average degree 2..4, "
                     f"cap {cap:.0e}; admission by footprint estimate within {budget_gb} GB, at most {max_batch} "
                     f"per batch",
-        "server": "continuous batching at batch granularity: each csm_batch_match takes every request that arrived "
-                  "while the previous one ran (a batch of one runs as csm_match); cpp_server: the overlapped C++ "
-                  "server on the same arrivals",
+        "server": "batched: continuous batching at batch granularity written over csm_batch_match, each batch "
+                  "takes every request that arrived while the previous one ran (a batch of one runs as csm_match); "
+                  "cpp_server: the library's server (csm_serve_*) on the same kind of arrivals, each request admitted "
+                  "into the running resident join kernel",
         "queries": len(keep),
         "screened_out": [i for i in range(nqueries) if i not in keep],
         "offered_qps": qps,
         "batched": main["batched"],
         "plain_fifo": main["plain_fifo"],
         "other_loads": more,
-        "cpp_server": {"server": "csm_serve_*: admission by footprint, batch k+1 built on a second stream while "
-                                 "batch k's persistent join runs", "loads": cpp},
+        "cpp_server": {"server": "csm_serve_*: a resident persistent join kernel; arriving requests are admitted "
+                                 "by footprint and slots, prepared (filter, candidate index) on a second stream beside "
+                                 "the running kernel and handed to it through a doorbell in pinned memory; each "
+                                 "result is published as its query's last task ends", "loads": cpp},
         "counts_equal_plain": not mism,
         "mismatches": mism,
     }

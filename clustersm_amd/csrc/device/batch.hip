@@ -220,6 +220,13 @@ __device__ inline bool aborted(const BArgs &a) {
 __device__ inline void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 constexpr uint32_t kMaxSpin = 1u << 24;
+// An idle workgroup's scheduling round reads the deques and counters of the
+// others with device-scope atomics; between rounds it sleeps
+// min(rounds idle, kMaxIdleRounds) x 16 x 64 cycles (up to about 60 us).
+// With a cap of 16 (about 8 us) the polling of several hundred idle
+// workgroups slowed the few busy ones' memory accesses by an order of
+// magnitude: one deep, narrow query ran 6x slower in the resident kernel.
+constexpr uint32_t kMaxIdleRounds = 128;
 
 // Deque lock: held by one thread for a few loads and stores, never across a
 // wait, and contended only by the owner and an occasional thief.
@@ -1063,10 +1070,6 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
 }
 
 // ---------------------------------------------------------------- resident mode
-template <typename T>
-__device__ inline T ld_sys(T *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 constexpr uint64_t kPollTicks = 200;  // host doorbell read at most every 2 us (100 MHz clock)
 
 // Thread 0: reads the host doorbell (one workgroup at a time, rate-limited)
@@ -1077,11 +1080,14 @@ __device__ void serve_admit(const BArgs &a) {
   const unsigned long long now = __builtin_amdgcn_s_memrealtime();
   const unsigned long long last = ld_rlx(&a.ctl->last_poll);
   if (now - last < kPollTicks || atomicCAS(&a.ctl->last_poll, last, now) != last) return;
-  const unsigned long long db = ld_sys(&a.sh->doorbell);
-  st_rlx(&a.ctl->stop_mirror, ld_sys(&a.sh->stop));
+  // relaxed reads while nothing is new: an acquire invalidates caches that
+  // the busy workgroups of this XCD are reading through
+  const unsigned long long db = __hip_atomic_load(&a.sh->doorbell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  st_rlx(&a.ctl->stop_mirror, __hip_atomic_load(&a.sh->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
   st_rlx(&a.ctl->db_mirror, db);
   if (db <= ld_rlx(&a.ctl->adm_seen) || atomicCAS(&a.ctl->adm_lock, 0u, 1u) != 0u) return;
-  acquire_agent();
+  // the host's ring entries were written before its release of the doorbell
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
   if (__hip_atomic_load(&a.ctl->exiting, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {  // the next launch admits
     __hip_atomic_store(&a.ctl->adm_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
@@ -1115,9 +1121,13 @@ __device__ void serve_admit(const BArgs &a) {
 // Thread 0: n tasks of query q are finished; the query's last publishes its
 // completion record (the counts of finished tasks are in its state already).
 __device__ void serve_tasks_done(const BArgs &a, uint32_t q, unsigned long long n) {
+  // release: this workgroup's counts precede its decrement; only the last
+  // decrement acquires (every other workgroup's counts, released with their
+  // decrements), so the other task ends invalidate no cache
   const unsigned long long prev =
-      __hip_atomic_fetch_add(&a.st[q].pending, 0ull - n, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&a.st[q].pending, 0ull - n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   if (prev != n) return;
+  acquire_agent();
   BState &st = a.st[q];
   SlotDone *d = &a.sh->done[q];
   const unsigned int gen = __hip_atomic_load(&st.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1135,6 +1145,9 @@ __device__ void serve_tasks_done(const BArgs &a, uint32_t q, unsigned long long 
 // as finished tasks, which keeps its task accounting exact.
 __device__ bool serve_take_root(const BArgs &a, Smem &sm, uint32_t &rec) {
   const unsigned long long T = a.tasks_per_query;
+  // no root left: decided with relaxed loads (an idle workgroup asks often,
+  // and an acquire would invalidate caches the busy ones read through)
+  if (ld_rlx(&a.ctl->root_next) >= ld_rlx(&a.ctl->adm_seen) * T) return false;
   for (;;) {
     const unsigned long long seen = __hip_atomic_load(&a.ctl->adm_seen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long i = ld_rlx(&a.ctl->root_next);
@@ -1210,17 +1223,13 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       if (__builtin_amdgcn_s_memrealtime() - t_start > a.deadline_ticks) atomicCAS(&a.ctl->abort, 0u, kAbortTime);
       sm.state = 1;
       const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-      // own deque, then a new root task, then the oldest task of another workgroup
+      // own deque, then a new root task, then the oldest task of another
+      // workgroup.  Resident mode keeps this order too: taking new queries'
+      // roots before the own deque's deeper tasks was tried, and it starved
+      // those tasks (a long query's bottom entries waited behind every later
+      // arrival) without lowering the median.
       uint32_t rec = kNil;
-      // resident mode: a newly admitted query's roots come before this
-      // workgroup's own older tasks (while its deque holds few tasks and so
-      // few pool blocks), so new queries overtake a long one instead of
-      // queueing behind it
-      if (a.serve && sm.dq_bottom - ld_rlx(&a.dq[wg].top) < 16 && serve_take_root(a, sm, rec) && idle) {
-        atomicAdd(&a.ctl->idle, ~0ull);
-        idle = false;
-      }
-      unsigned long long ent = rec != kNil || idle ? kNilEntry : deque_pop(a, sm, wg);  // an idle deque is empty
+      unsigned long long ent = idle ? kNilEntry : deque_pop(a, sm, wg);  // an idle deque is empty
       if (rec != kNil) {
       } else if (ent == kNilEntry && a.serve) {
         if (serve_take_root(a, sm, rec) && idle) {
@@ -1351,7 +1360,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
     if (state == 2) break;
     if (state == 1) {  // back off while the other workgroups produce work
       if (tid == 0) flush_acc(a, sm);
-      idle_rounds = min(idle_rounds + 1, 16u);
+      idle_rounds = min(idle_rounds + 1, kMaxIdleRounds);
       for (uint32_t i = 0; i < idle_rounds; ++i) __builtin_amdgcn_s_sleep(16);
       continue;
     }
@@ -1876,14 +1885,22 @@ csm_status ResidentJoin::launch() {
   return CSM_OK;
 }
 
-csm_status ResidentJoin::admit(const BatchRun &b, uint32_t i, hipStream_t s, uint32_t &slot) {
-  if (free_.empty()) return set_error(ctx_, CSM_E_BUSY, "no free slot");
+bool ResidentJoin::reserve(uint32_t &slot) {
+  if (free_.empty()) return false;
   slot = free_.back();
+  free_.pop_back();
+  return true;
+}
+
+csm_status ResidentJoin::fill(const BatchRun &b, uint32_t i, hipStream_t s, uint32_t slot) {
   BArgs &a = *static_cast<BArgs *>(args_);
   csm_status st = b.copy_descriptor(i, const_cast<BQuery *>(a.qs) + slot, s);
   if (st != CSM_OK) return st;
   CSM_HIP_TRY(ctx_, hipStreamSynchronize(s));
-  free_.pop_back();
+  return CSM_OK;
+}
+
+void ResidentJoin::ring(uint32_t slot) {
   busy_[slot] = 1;
   const uint32_t gen = ++gen_[slot];
   ServeShared *sh = static_cast<ServeShared *>(sh_);
@@ -1891,7 +1908,6 @@ csm_status ResidentJoin::admit(const BatchRun &b, uint32_t i, hipStream_t s, uin
   __atomic_store_n(&e.slot, slot, __ATOMIC_RELAXED);
   __atomic_store_n(&e.gen, gen, __ATOMIC_RELAXED);
   __atomic_store_n(&sh->doorbell, ++doorbell_, __ATOMIC_RELEASE);
-  return CSM_OK;
 }
 
 csm_status ResidentJoin::poll(std::vector<Done> &done) {
@@ -1910,6 +1926,17 @@ csm_status ResidentJoin::poll(std::vector<Done> &done) {
     BCtl hctl{};
     CSM_HIP_TRY(ctx_, hipMemcpyAsync(&hctl, a.ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s_));
     CSM_HIP_TRY(ctx_, hipStreamSynchronize(s_));
+    const char *tr = std::getenv("CSM_TRACE");
+    if (tr && *tr == '1') {
+      const double g = static_cast<double>(ctx_->num_cus * wgs_per_cu_) * 1e5;
+      std::fprintf(stderr,
+                   "csm resident: launch ended: tasks %llu items %llu rows_in %llu rows_out %llu steals %llu "
+                   "pool misses %llu; per workgroup: busy %.1f idle %.1f sync %.1f staging %.1f count %.1f "
+                   "extension %.1f task ends %.1f ms\n",
+                   hctl.tasks, hctl.items, hctl.rows_in, hctl.rows_out, hctl.steals, hctl.alloc_fail,
+                   hctl.busy_ticks / g, hctl.idle_ticks / g, hctl.sync_ticks / g, hctl.phase_ticks[0] / g,
+                   hctl.phase_ticks[1] / g, hctl.phase_ticks[2] / g, hctl.phase_ticks[3] / g);
+    }
     if (hctl.abort) {
       // stopped early: every admitted, unfinished query is recounted by the
       // caller, and the next launch starts after the last admission

@@ -81,10 +81,18 @@ class ResidentJoin {
   ResidentJoin(const ResidentJoin &) = delete;
   ResidentJoin &operator=(const ResidentJoin &) = delete;
   csm_status init(hipStream_t join_stream);
+  // Admission in three steps, so that a caller serialising the slot
+  // bookkeeping with poll() need not hold its lock across the copy:
+  // reserve() takes a free slot, fill() writes active query i of a prepared
+  // batch into the slot's descriptor (copied on stream s, which is
+  // synchronised), ring() hands the slot to the kernel through the
+  // doorbell; unreserve() returns a slot whose fill failed.  reserve, ring,
+  // unreserve, free_slots and poll share the slot state: one caller at a time.
   uint32_t free_slots() const { return static_cast<uint32_t>(free_.size()); }
-  // Admits active query i of a prepared batch: descriptor to a free slot
-  // (copied on stream s, which is synchronised), then the doorbell.
-  csm_status admit(const BatchRun &b, uint32_t i, hipStream_t s, uint32_t &slot);
+  bool reserve(uint32_t &slot);
+  csm_status fill(const BatchRun &b, uint32_t i, hipStream_t s, uint32_t slot);
+  void ring(uint32_t slot);
+  void unreserve(uint32_t slot) { free_.push_back(slot); }
   // Launches the kernel when admitted work waits and it is not running;
   // collects completed slots (their slots become free).
   csm_status poll(std::vector<Done> &done);

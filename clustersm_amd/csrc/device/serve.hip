@@ -18,9 +18,16 @@
 // group whose build runs out of memory is split in halves and retried.  The
 // resident kernel's frontier pool is min(budget / 4, 8 GiB).
 //
-// Threading: submit/poll/stop are safe from any thread; while a server runs,
-// its worker owns the context's streams, so the context must not be used for
-// other matching calls until csm_serve_stop.
+// Threading: submit/poll/stop are safe from any thread.  Two threads work
+// for the server: the preparation thread admits requests, prepares them and
+// rings the doorbell (it also frees finished groups and recounts what an
+// early kernel stop left unfinished); the completion thread polls the
+// kernel's completion records, relaunches the kernel when admitted work waits
+// and publishes results, so a result is never held back by a preparation in
+// progress.  The slot bookkeeping (ResidentJoin's slot state, owners, groups)
+// is under one lock, taken before the request-queue lock where both are held.
+// While a server runs, its threads own the context's streams, so the context
+// must not be used for other matching calls until csm_serve_stop.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -56,6 +63,7 @@ class Server {
     csm_query_plan plan;
     Clock::time_point arrival;
     uint64_t est;
+    Clock::time_point admitted, rung;  // taken from the queue; handed to the kernel
   };
   // Requests prepared together; their memory is released when the last one
   // has its result.
@@ -71,7 +79,8 @@ class Server {
     uint32_t i = 0;
   };
 
-  void loop();
+  void prep_loop();
+  void done_loop();
   std::vector<Req> admit();
   bool build(std::vector<Req> reqs);
   void publish(const Req &r, csm_result res);
@@ -87,10 +96,15 @@ class Server {
   std::vector<uint64_t> label_count_;  // data vertices per label
   double avg_degree_ = 0;
   uint64_t pool_bytes_ = 0;
+  // slot bookkeeping, under bk_
+  std::mutex bk_;
   std::unique_ptr<ResidentJoin> rj_;
   std::vector<Owner> owner_;               // per slot
   std::list<std::unique_ptr<Group>> groups_;
   uint64_t inflight_est_ = 0;
+  std::vector<Group *> to_release_;        // groups whose last result is published
+  std::vector<Owner> to_recount_;          // left unfinished by an early kernel stop
+  bool prep_done_ = false;                 // the preparation thread has exited
 
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
@@ -99,7 +113,7 @@ class Server {
   std::unordered_map<uint64_t, bool> known_;  // submitted and not yet polled
   uint64_t next_ticket_ = 1;
   bool stopping_ = false, running_ = false;
-  std::thread worker_;
+  std::thread worker_, completer_;
 };
 
 uint64_t Server::estimate(const csm_query_plan &p) const {
@@ -145,7 +159,8 @@ csm_status Server::start() {
   }
   owner_.assign(o_.max_inflight, Owner{});
   running_ = true;
-  worker_ = std::thread([this] { loop(); });
+  worker_ = std::thread([this] { prep_loop(); });
+  completer_ = std::thread([this] { done_loop(); });
   return CSM_OK;
 }
 
@@ -183,11 +198,13 @@ csm_status Server::poll(uint64_t ticket, int wait, csm_result *out) {
 void Server::stop() {
   {
     std::lock_guard<std::mutex> lk(mu_);
-    if (!running_ && !worker_.joinable()) return;
+    if (!running_ && !worker_.joinable() && !completer_.joinable()) return;
     stopping_ = true;
     cv_work_.notify_all();
   }
   if (worker_.joinable()) worker_.join();
+  if (completer_.joinable()) completer_.join();
+  if (rj_) rj_->shutdown();
   {
     std::lock_guard<std::mutex> lk(mu_);
     running_ = false;
@@ -202,6 +219,7 @@ void Server::stop() {
 // Requests to admit now, in arrival order, while slots and the budget allow
 // (the first always when nothing is in flight).
 std::vector<Server::Req> Server::admit() {
+  std::lock_guard<std::mutex> bk(bk_);
   std::lock_guard<std::mutex> lk(mu_);
   std::vector<Req> out;
   const uint64_t limit = o_.budget_bytes / 2 > pool_bytes_ ? o_.budget_bytes / 2 - pool_bytes_ : 0;
@@ -212,6 +230,7 @@ std::vector<Server::Req> Server::admit() {
     if ((!out.empty() || !groups_.empty()) && sum + r.est > limit) break;
     sum += r.est;
     out.push_back(r);
+    out.back().admitted = Clock::now();
     pending_.pop_front();
   }
   return out;
@@ -224,10 +243,12 @@ void Server::publish(const Req &r, csm_result res) {
   cv_done_.notify_all();
 }
 
+// Preparation thread: frees a group whose last result is published.
 void Server::release(Group *g) {
   g->run.reset();
   for (csm_query *q : g->queries) csm_query_free(ctx_, q);
   g->queries.clear();
+  std::lock_guard<std::mutex> bk(bk_);
   inflight_est_ -= g->est;
   for (auto it = groups_.begin(); it != groups_.end(); ++it)
     if (it->get() == g) {
@@ -236,9 +257,12 @@ void Server::release(Group *g) {
     }
 }
 
+// Publishes query i's result; the group goes to the preparation thread for
+// freeing once all of its results are out.
 void Server::finish_one(Group *g, uint32_t i, csm_result res) {
   publish(g->reqs[i], res);
-  if (--g->outstanding == 0) release(g);
+  std::lock_guard<std::mutex> bk(bk_);
+  if (--g->outstanding == 0) to_release_.push_back(g);
 }
 
 // Prepares the requests' queries, filter and candidate indexes on the
@@ -274,19 +298,20 @@ bool Server::build(std::vector<Req> reqs) {
     }
     Group *gp = g.get();
     for (const Req &r : reqs) gp->est += r.est;
-    inflight_est_ += gp->est;
-    groups_.push_back(std::move(g));
-    gp->outstanding = static_cast<uint32_t>(reqs.size());
-    if (st != CSM_OK) {  // the whole group fails with the status
-      for (uint32_t i = 0; i < reqs.size(); ++i) {
-        csm_result r{};
-        r.status = st;
-        finish_one(gp, i, r);
-      }
-      return true;
+    {
+      std::lock_guard<std::mutex> bk(bk_);
+      inflight_est_ += gp->est;
+      groups_.push_back(std::move(g));
+      // one count more than the results: the group stays while it is admitted
+      gp->outstanding = static_cast<uint32_t>(reqs.size()) + 1;
     }
     for (uint32_t i = 0; i < reqs.size(); ++i) {
       csm_result r{};
+      if (st != CSM_OK) {  // the whole group fails with the status
+        r.status = st;
+        finish_one(gp, i, r);
+        continue;
+      }
       r.ms_filter = gp->run->ms_filter();
       if (!gp->run->active(i)) {  // answered without a join
         const uint64_t cap = o_.cap ? o_.cap : ~0ull;
@@ -297,29 +322,72 @@ bool Server::build(std::vector<Req> reqs) {
         continue;
       }
       uint32_t slot = 0;
-      st = rj_->admit(*gp->run, i, s_prep_, slot);
-      if (st != CSM_OK) {
-        r.status = st;
+      bool got;
+      {
+        std::lock_guard<std::mutex> bk(bk_);
+        got = rj_->reserve(slot);
+      }
+      // admit() counted free slots, and only completions free more
+      const csm_status fs = got ? rj_->fill(*gp->run, i, s_prep_, slot) : set_error(ctx_, CSM_E_BUSY, "no free slot");
+      if (fs != CSM_OK) {
+        if (got) {
+          std::lock_guard<std::mutex> bk(bk_);
+          rj_->unreserve(slot);
+        }
+        r.status = fs;
         finish_one(gp, i, r);
         continue;
       }
-      owner_[slot] = Owner{gp, i};
+      std::lock_guard<std::mutex> bk(bk_);
+      owner_[slot] = Owner{gp, i};  // before the kernel can finish it
+      gp->reqs[i].rung = Clock::now();
+      rj_->ring(slot);
     }
+    std::lock_guard<std::mutex> bk(bk_);
+    if (--gp->outstanding == 0) to_release_.push_back(gp);
     return true;
   }
   return false;
 }
 
-void Server::loop() {
+// Preparation thread: frees finished groups, recounts what an early kernel
+// stop left unfinished, admits and prepares arrived requests.
+void Server::prep_loop() {
   hipSetDevice(ctx_->device);
-  const uint64_t cap = o_.cap ? o_.cap : ~0ull;
   const char *tr = std::getenv("CSM_TRACE");
   const bool trace = tr && *tr == '1';
   const Clock::time_point t_start = Clock::now();
   auto ms = [&]() { return std::chrono::duration<double, std::milli>(Clock::now() - t_start).count(); };
-  std::vector<ResidentJoin::Done> done;
   for (;;) {
     bool worked = false;
+    std::vector<Group *> rel;
+    std::vector<Owner> rec;
+    {
+      std::lock_guard<std::mutex> bk(bk_);
+      rel.swap(to_release_);
+      rec.swap(to_recount_);
+    }
+    for (const Owner &o : rec) {  // the plain path, exact
+      csm_match_opts mo{};
+      mo.cap = o_.cap;
+      mo.refine = o_.refine;
+      mo.reuse_filter = 1;
+      csm_result pr{};
+      const csm_status rs = run_join(ctx_, g_, o.g->queries[o.i], mo, pr);
+      csm_result r{};
+      r.ms_filter = o.g->run->ms_filter();
+      r.count = pr.count;
+      r.capped = pr.capped;
+      r.items = pr.items;
+      r.status = rs;
+      r.path = CSM_PATH_BATCH_ABORT;
+      finish_one(o.g, o.i, r);
+      worked = true;
+    }
+    for (Group *gp : rel) {
+      release(gp);
+      worked = true;
+    }
     std::vector<Req> reqs = admit();
     if (!reqs.empty()) {
       const double t0 = ms();
@@ -327,15 +395,66 @@ void Server::loop() {
       worked |= build(std::move(reqs));
       if (trace) std::fprintf(stderr, "csm serve %.2f ms: %zu admitted, prepared in %.2f ms\n", ms(), n, ms() - t0);
     }
+    {
+      std::lock_guard<std::mutex> bk(bk_);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (stopping_ && pending_.empty() && groups_.empty() && to_release_.empty() && to_recount_.empty()) {
+        prep_done_ = true;
+        break;
+      }
+    }
+    if (!worked) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_work_.wait_for(lk, std::chrono::milliseconds(1));
+    }
+  }
+}
+
+// Completion thread: polls the kernel's completion records (relaunching the
+// kernel when admitted work waits) and publishes results.
+void Server::done_loop() {
+  hipSetDevice(ctx_->device);
+  const uint64_t cap = o_.cap ? o_.cap : ~0ull;
+  const char *tr = std::getenv("CSM_TRACE");
+  const bool trace = tr && *tr == '1';
+  const Clock::time_point t_start = Clock::now();
+  auto ms = [&]() { return std::chrono::duration<double, std::milli>(Clock::now() - t_start).count(); };
+  std::vector<ResidentJoin::Done> done;
+  std::vector<Owner> owners, failed;
+  for (;;) {
     done.clear();
-    const bool was_running = rj_->running();
-    csm_status st = rj_->poll(done);
-    if (trace && (done.size() || was_running != rj_->running()))
+    owners.clear();
+    failed.clear();
+    bool was_running, now_running, exit_now;
+    csm_status st;
+    {
+      std::lock_guard<std::mutex> bk(bk_);
+      was_running = rj_->running();
+      st = rj_->poll(done);
+      for (const ResidentJoin::Done &d : done) {
+        owners.push_back(owner_[d.slot]);
+        owner_[d.slot] = Owner{};
+      }
+      if (st != CSM_OK)  // the kernel cannot be launched: fail what is in flight
+        for (Owner &o : owner_)
+          if (o.g) {
+            failed.push_back(o);
+            o = Owner{};
+          }
+      now_running = rj_->running();
+      exit_now = prep_done_;
+    }
+    if (trace && (done.size() || was_running != now_running))
       std::fprintf(stderr, "csm serve %.2f ms: %zu done, kernel %s\n", ms(), done.size(),
-                   rj_->running() ? (was_running ? "running" : "launched") : "stopped");
-    for (const ResidentJoin::Done &d : done) {
-      const Owner o = owner_[d.slot];
-      owner_[d.slot] = Owner{};
+                   now_running ? (was_running ? "running" : "launched") : "stopped");
+    for (size_t j = 0; j < done.size(); ++j) {
+      const ResidentJoin::Done &d = done[j];
+      const Owner o = owners[j];
+      if (d.recount) {  // the resident kernel stopped early: recounted by the preparation thread
+        std::lock_guard<std::mutex> bk(bk_);
+        to_recount_.push_back(o);
+        continue;
+      }
       csm_result r{};
       r.ms_filter = o.g->run->ms_filter();
       r.count = std::min(d.count, cap);
@@ -343,37 +462,26 @@ void Server::loop() {
       r.items = d.items;
       r.partial_rows = d.rows;
       r.path = CSM_PATH_BATCH;
-      if (d.recount) {  // the resident kernel stopped early: the plain path, exact
-        csm_match_opts mo{};
-        mo.cap = o_.cap;
-        mo.refine = o_.refine;
-        mo.reuse_filter = 1;
-        csm_result pr{};
-        const csm_status rs = run_join(ctx_, g_, o.g->queries[o.i], mo, pr);
-        r.count = pr.count;
-        r.capped = pr.capped;
-        r.items = pr.items;
-        r.status = rs;
-        r.path = CSM_PATH_BATCH_ABORT;
+      if (trace) {
+        const Req &q = o.g->reqs[o.i];
+        auto d_ms = [](Clock::time_point a, Clock::time_point b) {
+          return std::chrono::duration<double, std::milli>(b - a).count();
+        };
+        std::fprintf(stderr, "csm serve %.2f ms: ticket %llu queued %.2f prepared %.2f joined %.2f ms, %llu items\n",
+                     ms(), static_cast<unsigned long long>(q.ticket), d_ms(q.arrival, q.admitted),
+                     d_ms(q.admitted, q.rung), d_ms(q.rung, Clock::now()), static_cast<unsigned long long>(d.items));
       }
       finish_one(o.g, o.i, r);
-      worked = true;
     }
-    if (st != CSM_OK) {  // the kernel cannot be launched: fail what is in flight
-      for (uint32_t slot = 0; slot < owner_.size(); ++slot) {
-        if (!owner_[slot].g) continue;
-        csm_result r{};
-        r.status = st;
-        const Owner o = owner_[slot];
-        owner_[slot] = Owner{};
-        finish_one(o.g, o.i, r);
-      }
+    for (const Owner &o : failed) {
+      csm_result r{};
+      r.status = st;
+      finish_one(o.g, o.i, r);
     }
-    std::unique_lock<std::mutex> lk(mu_);
-    if (stopping_ && pending_.empty() && groups_.empty()) break;
-    if (!worked) cv_work_.wait_for(lk, std::chrono::microseconds(50));
+    if (!done.empty() || !failed.empty()) cv_work_.notify_one();  // slots and memory freed
+    if (exit_now) break;
+    if (done.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
-  rj_->shutdown();
 }
 
 }  // namespace csm
