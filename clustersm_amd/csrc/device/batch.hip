@@ -147,7 +147,8 @@ struct BCtl {
   unsigned long long busy_ticks;   // real-time ticks inside tasks, summed over workgroups
   unsigned long long idle_ticks;   // real-time ticks without a task
   unsigned long long sync_ticks;   // real-time ticks in deque and free-ring operations
-  unsigned long long phase_ticks[4];  // rows staged + drivers, count tiles, extension tiles, task ends
+  unsigned long long phase_ticks[7];  // rows staged + drivers, count tiles, extension tiles, task ends;
+                                      // staging's parts: rows, list lookups
   unsigned long long alloc_fail;   // consecutive failed allocations from the free ring
   unsigned int abort;              // kAbortPool, kAbortTime or kAbortBadTask once the kernel stops early
   unsigned int overflow;           // a deque was full (never expected)
@@ -340,7 +341,8 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     unsigned long long *c64[] = {&c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out,
                                  &c.bytes_ref, &c.bytes_own, &c.tasks,
                                  &c.steals, &c.fails, &c.busy_ticks, &c.idle_ticks, &c.sync_ticks, &c.phase_ticks[0],
-                                 &c.phase_ticks[1], &c.phase_ticks[2], &c.phase_ticks[3], &c.alloc_fail,
+                                 &c.phase_ticks[1], &c.phase_ticks[2], &c.phase_ticks[3], &c.phase_ticks[4],
+                                 &c.phase_ticks[5], &c.phase_ticks[6], &c.alloc_fail,
                                  &c.late_recs, &c.bad_recs};
     for (unsigned long long *f : c64) st_init(f, 0ull);
     st_init(&c.abort, 0u);
@@ -368,6 +370,7 @@ struct Smem {
   // one anchor of different rows hit consecutive banks: begin
   uint64_t abeg[kMaxA][kRowBatch];
   uint32_t alen[kMaxA][kRowBatch];  // and length
+  int32_t ahub[kMaxA][kRowBatch];   // and the anchor vertex's hub index (-1: none)
   uint64_t dbeg[kRowBatch];         // driver list start (after a resume skip)
   // fused count, when the last position's parent is an earlier column: per
   // row, |N(x) ∩ C(u_last)| minus the row's own members of that list, and the list
@@ -403,7 +406,7 @@ struct Smem {
   uint32_t acc_q;
   unsigned long long acc_count, acc_items, acc_rows;
   unsigned long long sync_ticks;
-  unsigned long long phase[4];
+  unsigned long long phase[7];  // staging, count tiles, extension tiles, task ends; staging's rows, lists
   int flag;                         // 0 go on, 1 output bound reached, 2 pool empty: yield with continuation
   int state;                        // loop control: 0 work, 1 idle, 2 exit
   int stop;                         // broadcast: the task's query is finished or the kernel aborts
@@ -714,12 +717,12 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   __syncthreads();
   uint64_t tp0 = __builtin_amdgcn_s_memrealtime();
   for (; row0 < nrows; row0 += kRowBatch) {
+    // the early-stop check and the batch's rows in one round trip (rows of a
+    // batch that turns out stopped are read in bounds and dropped)
     if (tid == 0) {
       sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
       if (sm.stop && ld_done(a, t.q) == kRunning) __hip_atomic_store(&a.st[t.q].lost, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // cut short by an early stop
     }
-    __syncthreads();
-    if (sm.stop) break;
     const uint32_t nb = min(kRowBatch, nrows - row0);
     for (uint32_t i = tid; i < nb * k; i += kThreads)
       s_rows[i] = k == 1    ? sm.roots[row0 + i]
@@ -727,15 +730,44 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
                                                __HIP_MEMORY_SCOPE_AGENT)
                            : grows[static_cast<uint64_t>(row0) * k + i];
     __syncthreads();
-    // candidate lists of the kept anchors, one (row, anchor) pair per thread
-    for (uint32_t p = tid; p < nb * nkept; p += kThreads) {
-      const uint32_t r = p / nkept, x = p - r * nkept;
+    if (sm.stop) break;
+    uint64_t tq = 0;  // staging sub-phases (thread 0): rows, list lookups, drivers and scan
+    if (tid == 0) {
+      tq = __builtin_amdgcn_s_memrealtime();
+      sm.phase[4] += tq - tp0;
+    }
+    // candidate lists of the kept anchors, one (row, anchor) pair per thread,
+    // with each anchor vertex's hub index, and for a fused count the last
+    // position's list as one more pair per row: every load here depends on
+    // the row alone, so they are all in flight at once
+    const uint32_t nslot = nkept + (fused_rows ? 1u : 0u);
+    for (uint32_t p = tid; p < nb * nslot; p += kThreads) {
+      const uint32_t r = p / nslot, x = p - r * nslot;
+      const uint32_t *row = s_rows + r * k;
       uint64_t b0, e0;
-      Q.ci.list(sm.lv, x, s_rows[r * k + sm.lv.anchor[x]], b0, e0);
-      sm.abeg[x][r] = b0;
-      sm.alen[x][r] = static_cast<uint32_t>(e0 - b0);
+      if (x < nkept) {
+        const uint32_t v = row[sm.lv.anchor[x]];
+        Q.ci.list(sm.lv, x, v, b0, e0);
+        sm.abeg[x][r] = b0;
+        sm.alen[x][r] = static_cast<uint32_t>(e0 - b0);
+        sm.ahub[x][r] = a.probe.hub_id[v];
+        if (x == 0) {  // byte accounting: the reference's extension of this row
+          // reads the row and its parent's offsets and scans the parent's
+          // adjacency (id + candidate word per entry)
+          const uint32_t xp = row[sm.lv.parent];
+          b_ref += 4ull * k + 8 + 12ull * (a.probe.off[xp + 1] - a.probe.off[xp]);
+        }
+      } else {
+        Q.ci.list(sm.lv_last, 0, row[sm.lv_last.parent], b0, e0);
+        sm.fbeg[r] = b0;
+        sm.flen[r] = static_cast<uint32_t>(e0 - b0);
+      }
     }
     __syncthreads();
+    if (tid == 0) {
+      const uint64_t t = __builtin_amdgcn_s_memrealtime();
+      sm.phase[5] += t - tq;
+    }
     // each row: the shortest list drives
     uint64_t len = 0;
     if (static_cast<uint32_t>(tid) < nb) {
@@ -769,36 +801,53 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       // against its adjacency bitmap (the driver's start is taken above)
       for (uint32_t x = 0; x < nkept; ++x) {
         if (x == bi) continue;
-        const int32_t h = a.probe.hub_id[row[sm.lv.anchor[x]]];
+        const int32_t h = sm.ahub[x][tid];
         if (h >= 0) sm.abeg[x][tid] = kHubTag | static_cast<uint64_t>(h);
       }
-      {  // byte accounting: the reference's extension of this row reads the
-         // row and its parent's offsets and scans the parent's adjacency (id
-         // + candidate word per entry); this kernel reads the row and one
-         // rank word, prefix, list start and length per list it looks up
-        const uint32_t xp = row[sm.lv.parent];
-        b_ref += 4ull * k + 8 + 12ull * (a.probe.off[xp + 1] - a.probe.off[xp]);
-        b_own += 4ull * k + 28ull * (nanchor + (fused_rows ? 1 : 0));
-      }
-      if (fused_rows) {  // the row's part of the last position's count
+      // this kernel reads the row and one rank word, prefix, list start and
+      // length per list it looks up
+      b_own += 4ull * k + 28ull * (nanchor + (fused_rows ? 1 : 0));
+      if (fused_rows) {  // the row's part of the last position's count: the
+        // list minus the row's own members of it, searched in lockstep
         const uint32_t xv = row[sm.lv_last.parent];
-        uint64_t lb, le;
-        Q.ci.list(sm.lv_last, 0, xv, lb, le);
-        uint32_t c = static_cast<uint32_t>(le - lb);
-        for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) {
-          const uint32_t col = sm.lv_last.inj[x];
-          if (col == k) continue;
-          const uint32_t v = row[col];
-          if (v != xv && in_list(Q.ci.cn_adj, lb, static_cast<uint32_t>(le - lb), v, probe_loads)) --c;
+        const uint64_t lb = sm.fbeg[tid];
+        const uint32_t ln = sm.flen[tid];
+        uint32_t c = ln;
+        for (uint32_t x0 = 0; x0 < sm.lv_last.ninj; x0 += kIlp) {
+          uint64_t lo[kIlp];
+          uint32_t n[kIlp], v[kIlp];
+          bool f[kIlp];
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            const uint32_t x = x0 + j;
+            const uint32_t col = x < sm.lv_last.ninj ? sm.lv_last.inj[x] : k;
+            v[j] = col < k ? row[col] : 0u;
+            lo[j] = lb;
+            n[j] = col < k && v[j] != xv ? ln : 0u;  // column k is this level's new id: per item
+          }
+          lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, v, f, probe_loads);
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) c -= f[j] ? 1u : 0u;
         }
         sm.fbase[tid] = c;
-        sm.fbeg[tid] = lb;
-        sm.flen[tid] = static_cast<uint32_t>(le - lb);
       }
       if (row_counted) {  // one anchor: the row's survivors are its list minus the row's own ids
         uint64_t surv = len;
-        for (uint32_t x = 0; x < sm.lv.ninj; ++x)
-          if (in_list(Q.ci.cn_adj, bbeg, static_cast<uint32_t>(len), row[sm.lv.inj[x]], probe_loads)) --surv;
+        for (uint32_t x0 = 0; x0 < sm.lv.ninj; x0 += kIlp) {
+          uint64_t lo[kIlp];
+          uint32_t n[kIlp], v[kIlp];
+          bool f[kIlp];
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) {
+            const uint32_t x = x0 + j;
+            v[j] = x < sm.lv.ninj ? row[sm.lv.inj[x]] : 0u;
+            lo[j] = bbeg;
+            n[j] = x < sm.lv.ninj ? static_cast<uint32_t>(len) : 0u;
+          }
+          lockstep_find<kIlp>(Q.ci.cn_adj, lo, n, v, f, probe_loads);
+#pragma unroll
+          for (uint32_t j = 0; j < kIlp; ++j) surv -= f[j] ? 1u : 0u;
+        }
         contrib += surv * (fused_rows ? sm.fbase[tid] : 1u);
         items_seen += len;
         len = 0;
@@ -1216,7 +1265,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
     sm.acc_count = sm.acc_items = sm.acc_rows = 0;
     sm.sync_ticks = 0;
     atomicAdd(&a.ctl->started, 1ull);
-    sm.phase[0] = sm.phase[1] = sm.phase[2] = sm.phase[3] = 0;
+    for (int i = 0; i < 7; ++i) sm.phase[i] = 0;
   }
   for (;;) {
     if (tid == 0) {
@@ -1421,7 +1470,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
     atomicAdd(&a.ctl->busy_ticks, static_cast<unsigned long long>(busy));
     atomicAdd(&a.ctl->idle_ticks, static_cast<unsigned long long>(idle_t));
     atomicAdd(&a.ctl->sync_ticks, sm.sync_ticks);
-    for (int i = 0; i < 4; ++i) atomicAdd(&a.ctl->phase_ticks[i], sm.phase[i]);
+    for (int i = 0; i < 7; ++i) atomicAdd(&a.ctl->phase_ticks[i], sm.phase[i]);
     atomicAdd(&a.ctl->items, n_items);
     atomicAdd(&a.ctl->rows_in, n_rows_in);
     atomicAdd(&a.ctl->rows_out, n_rows_out);
@@ -1701,9 +1750,10 @@ csm_status BatchRun::finish(csm_result *results) {
                  hctl.rows_out, hctl.steals, hctl.abort, hctl.overflow, hctl.fails, hctl.busy_ticks / 1e5 / grid,
                  hctl.idle_ticks / 1e5 / grid, hctl.sync_ticks / 1e5 / grid);
     std::fprintf(stderr,
-                 "csm batch: per workgroup: staging %.1f ms, count tiles %.1f ms, extension tiles %.1f ms, task ends %.1f ms\n",
-                 hctl.phase_ticks[0] / 1e5 / grid, hctl.phase_ticks[1] / 1e5 / grid, hctl.phase_ticks[2] / 1e5 / grid,
-                 hctl.phase_ticks[3] / 1e5 / grid);
+                 "csm batch: per workgroup: staging %.1f ms (rows %.1f, list lookups %.1f), count tiles %.1f ms, "
+                 "extension tiles %.1f ms, task ends %.1f ms\n",
+                 hctl.phase_ticks[0] / 1e5 / grid, hctl.phase_ticks[4] / 1e5 / grid, hctl.phase_ticks[5] / 1e5 / grid,
+                 hctl.phase_ticks[1] / 1e5 / grid, hctl.phase_ticks[2] / 1e5 / grid, hctl.phase_ticks[3] / 1e5 / grid);
   }
   if (hctl.late_recs || hctl.bad_recs || hctl.abort == kAbortBadTask)
     std::fprintf(stderr,

@@ -332,12 +332,15 @@ csm_status csm_batch_autotune(csm_ctx *ctx, const csm_graph *g, csm_query *const
                               const csm_batch_opts *base, uint32_t sample, csm_batch_opts *out);
 
 /* ----------------------------------------------------------------- serving
- * No reference counterpart.  A request loop keeps the graph resident, admits
- * queries by their estimated intermediate-result footprint against a memory
- * budget and batches the admitted ones while others run. */
+ * No reference counterpart (runClusterSM.sh:9-13 runs one process per query).
+ * The graph and one persistent join kernel stay resident; requests are
+ * admitted in arrival order while query slots are free and their estimated
+ * footprints fit the budget, prepared beside the running kernel and handed to
+ * it through a doorbell in pinned memory; each result is published as soon
+ * as its query's last task ends. */
 typedef struct csm_serve_opts {
   uint64_t budget_bytes;   /* admission budget for intermediate results */
-  uint32_t max_inflight;   /* queries on the device at once */
+  uint32_t max_inflight;   /* query slots of the resident kernel (0 = 256, at most 256) */
   uint32_t queue_capacity; /* pending submissions before CSM_E_BUSY */
   uint64_t cap;            /* per-query cap */
   uint32_t refine;
