@@ -139,7 +139,6 @@ struct BCtl {
   unsigned long long items;        // (row, list entry) pairs examined
   unsigned long long rows_in;      // rows read
   unsigned long long rows_out;     // rows written
-  unsigned long long bytes_ref;    // the reference algorithm's bytes for the rows processed (see finish())
   unsigned long long bytes_own;    // this kernel's own logical bytes
   unsigned long long tasks;
   unsigned long long steals;
@@ -339,7 +338,7 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
     st_init(&c.adm_lock, 0u);
     st_init(&c.exiting, 0u);
     unsigned long long *c64[] = {&c.idle, &c.started, &c.items, &c.rows_in, &c.rows_out,
-                                 &c.bytes_ref, &c.bytes_own, &c.tasks,
+                                 &c.bytes_own, &c.tasks,
                                  &c.steals, &c.fails, &c.busy_ticks, &c.idle_ticks, &c.sync_ticks, &c.phase_ticks[0],
                                  &c.phase_ticks[1], &c.phase_ticks[2], &c.phase_ticks[3], &c.phase_ticks[4],
                                  &c.phase_ticks[5], &c.phase_ticks[6], &c.alloc_fail,
@@ -356,6 +355,7 @@ constexpr uint64_t kHubTag = 1ull << 63;  // abeg entry of a hub anchor: tag | h
 constexpr uint32_t kMaxA = 4;    // anchors whose candidate lists are kept per row
 constexpr uint32_t kMaxOut = 8;  // most full output blocks a task holds before it yields
 constexpr uint32_t kCache = 16;  // free blocks a workgroup keeps for itself
+constexpr uint32_t kDefaultBlocksPerWg = 16;  // pool blocks per workgroup the default block size keeps
 
 static_assert(kRowBatch <= 256 && kRowBatch <= kThreads, "row index in a batch is a byte");
 
@@ -551,6 +551,21 @@ __device__ inline uint32_t row_of(const Smem &sm, uint32_t nb, uint64_t it) {
   return lo;
 }
 
+template <uint32_t N>
+__device__ inline void rows_of(const Smem &sm, uint32_t nb, const uint64_t (&it)[N], const bool (&act)[N],
+                               uint32_t (&r)[N]) {
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) r[j] = 0;
+#pragma unroll
+  for (uint32_t step = kRowBatch / 2; step > 0; step >>= 1) {
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+      const uint32_t c = r[j] + step;
+      if (act[j] && c < nb && sm.pre[c] <= it[j]) r[j] = c;
+    }
+  }
+}
+
 // Membership of v[j] in the sorted lists cn[lo[j] .. lo[j] + n[j]), all
 // searches advancing together so their loads overlap.  n[j] = 0: not found.
 // `loads` gains the number of list entries read.
@@ -580,15 +595,6 @@ __device__ inline void lockstep_find(const uint32_t *cn, uint64_t (&lo)[N], uint
     found[j] = has[j] && cn[lo[j]] == v[j];
     loads += has[j] ? 1 : 0;
   }
-}
-
-__device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint32_t len, uint32_t v, unsigned long long &loads) {
-  uint64_t lo[1] = {beg};
-  uint32_t n[1] = {len};
-  const uint32_t vv[1] = {v};
-  bool f[1];
-  lockstep_find<1>(cn, lo, n, vv, f, loads);
-  return f[0];
 }
 
 // Thread 0: writes task t into record `rec` under a new tag and returns the
@@ -651,7 +657,7 @@ __device__ uint32_t read_rec(const BArgs &a, unsigned long long ent, Task &t, bo
 // Process one task; returns with all threads converged.
 __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_t wg, unsigned long long &n_items,
                              unsigned long long &n_rows_in, unsigned long long &n_rows_out,
-                             unsigned long long &b_ref, unsigned long long &b_own) {
+                             unsigned long long &b_own) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const Task t = sm.task;
   const BQuery &Q = sm.q;
@@ -744,23 +750,15 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     for (uint32_t p = tid; p < nb * nslot; p += kThreads) {
       const uint32_t r = p / nslot, x = p - r * nslot;
       const uint32_t *row = s_rows + r * k;
-      uint64_t b0, e0;
       if (x < nkept) {
-        const uint32_t v = row[sm.lv.anchor[x]];
-        Q.ci.list(sm.lv, x, v, b0, e0);
-        sm.abeg[x][r] = b0;
-        sm.alen[x][r] = static_cast<uint32_t>(e0 - b0);
-        sm.ahub[x][r] = a.probe.hub_id[v];
-        if (x == 0) {  // byte accounting: the reference's extension of this row
-          // reads the row and its parent's offsets and scans the parent's
-          // adjacency (id + candidate word per entry)
-          const uint32_t xp = row[sm.lv.parent];
-          b_ref += 4ull * k + 8 + 12ull * (a.probe.off[xp + 1] - a.probe.off[xp]);
-        }
+        const CnLoc L = Q.ci.locate(sm.lv, x, row[sm.lv.anchor[x]]);
+        sm.abeg[x][r] = L.start;
+        sm.alen[x][r] = L.len;
+        sm.ahub[x][r] = L.hub;
       } else {
-        Q.ci.list(sm.lv_last, 0, row[sm.lv_last.parent], b0, e0);
-        sm.fbeg[r] = b0;
-        sm.flen[r] = static_cast<uint32_t>(e0 - b0);
+        const CnLoc L = Q.ci.locate(sm.lv_last, 0, row[sm.lv_last.parent]);
+        sm.fbeg[r] = L.start;
+        sm.flen[r] = L.len;
       }
     }
     __syncthreads();
@@ -782,11 +780,10 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         }
       }
       for (uint32_t x = kMaxA; x < nanchor; ++x) {
-        uint64_t b0, e0;
-        Q.ci.list(sm.lv, x, row[sm.lv.anchor[x]], b0, e0);
-        if (e0 - b0 < best) {
-          best = e0 - b0;
-          bbeg = b0;
+        const CnLoc L = Q.ci.locate(sm.lv, x, row[sm.lv.anchor[x]]);
+        if (L.len < best) {
+          best = L.len;
+          bbeg = L.start;
           bi = x;
         }
       }
@@ -804,9 +801,8 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         const int32_t h = sm.ahub[x][tid];
         if (h >= 0) sm.abeg[x][tid] = kHubTag | static_cast<uint64_t>(h);
       }
-      // this kernel reads the row and one rank word, prefix, list start and
-      // length per list it looks up
-      b_own += 4ull * k + 28ull * (nanchor + (fused_rows ? 1 : 0));
+      // this kernel reads the row and two 16-byte locators per list it looks up
+      b_own += 4ull * k + 32ull * (nanchor + (fused_rows ? 1 : 0));
       if (fused_rows) {  // the row's part of the last position's count: the
         // list minus the row's own members of it, searched in lockstep
         const uint32_t xv = row[sm.lv_last.parent];
@@ -869,18 +865,16 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     const uint64_t step = static_cast<uint64_t>(kThreads) * ilp;
     for (uint64_t base = 0; base < total; base += step) {
       uint32_t w[kIlp], ri[kIlp];
+      uint64_t its[kIlp];
       bool ok[kIlp];
 #pragma unroll
       for (uint32_t j = 0; j < kIlp; ++j) {
-        const uint64_t it = base + j * kThreads + tid;
-        ok[j] = j < ilp && it < total;
-        ri[j] = 0;
-        w[j] = 0;
-        if (ok[j]) {
-          ri[j] = row_of(sm, nb, it);
-          w[j] = Q.ci.cn_adj[sm.dbeg[ri[j]] + (it - sm.pre[ri[j]])];
-        }
+        its[j] = base + j * kThreads + tid;
+        ok[j] = j < ilp && its[j] < total;
       }
+      rows_of<kIlp>(sm, nb, its, ok, ri);
+#pragma unroll
+      for (uint32_t j = 0; j < kIlp; ++j) w[j] = ok[j] ? Q.ci.cn_adj[sm.dbeg[ri[j]] + (its[j] - sm.pre[ri[j]])] : 0u;
 #pragma unroll
       for (uint32_t j = 0; j < kIlp; ++j) items_seen += ok[j] ? 1 : 0;
       // injectivity against rows in LDS
@@ -947,7 +941,12 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
 #pragma unroll
           for (uint32_t j = 0; j < kIlp; ++j) {
             lb[j] = le[j] = 0;
-            if (ok[j]) Q.ci.list(sm.lv_last, 0, w[j], lb[j], le[j]);
+            if (ok[j]) {
+              const CnLoc L = Q.ci.locate(sm.lv_last, 0, w[j]);
+              lb[j] = L.start;
+              le[j] = L.start + L.len;
+              probe_loads += 8;  // two 16-byte locators, in 4-byte units
+            }
             contrib += le[j] - lb[j];
           }
           for (uint32_t x = 0; x < sm.lv_last.ninj; ++x) {
@@ -1065,14 +1064,19 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     if (sm.flag) break;
     __syncthreads();
   }
-  // finish: publish rows, push continuation and blocks, free the input;
-  // every lane's row stores complete before the fence's L2 write-back
+  // finish: push continuation and blocks, free the input.  Every lane's row
+  // stores are acknowledged by the L2 (vmcnt counts stores on gfx9) before
+  // the barriers below, so they sit in this XCD's L2 when thread 0 pushes.
+  // No fence here: an agent-scope fence writes back and invalidates the
+  // whole L2, and at one per task it dominated the kernel (1.4x slower).
+  // Rows reach other XCDs only through the release stores that make them
+  // visible: publishing deque entries (deque_publish) and returning blocks
+  // to the shared free ring (free_push), each an L2 write-back after these
+  // stores; the taker acquires (steal, free_pop) before reading.
   __builtin_amdgcn_s_waitcnt(0);
-  __threadfence();
   const unsigned long long tot_contrib = block_reduce(contrib, sm.red);
   const unsigned long long tot_items = block_reduce(items_seen, sm.red);
   b_own += 4ull * items_seen + 4ull * probe_loads + 8ull * hub_probes;  // list entries, searches, bitmap words
-  b_ref += 4ull * kout * rows_written;               // rows written (tid 0 holds the tile counts)
   b_own += 4ull * kout * rows_written;
   if (tid == 0) {
     const bool live = ld_done(a, t.q) == kRunning && !aborted(a);
@@ -1244,7 +1248,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
   const uint64_t nroot = static_cast<uint64_t>(a.nqueries) * a.tasks_per_query;
   uint64_t busy = 0;
   unsigned long long n_items = 0, n_rows_in = 0, n_rows_out = 0, n_tasks = 0, n_steals = 0;
-  unsigned long long b_ref = 0, b_own = 0;  // per thread, summed at exit
+  unsigned long long b_own = 0;  // per thread, summed at exit
   uint32_t rng = 0x9E3779B9u ^ (wg * 0x85EBCA6Bu + 1u);
   bool idle = false;  // counted in ctl->idle
   uint32_t idle_rounds = 0;
@@ -1443,7 +1447,7 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
       sm.staged_gen = sm.task_gen;
     }
     if (!sm.stop) {
-      process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out, b_ref, b_own);
+      process_task(a, sm, s_rows, wg, n_items, n_rows_in, n_rows_out, b_own);
       if (sm.flag == 2)  // pool empty: give other workgroups time to free blocks
         for (int i = 0; i < 16; ++i) __builtin_amdgcn_s_sleep(32);
     } else if (tid == 0) {
@@ -1460,11 +1464,9 @@ __global__ __launch_bounds__(kThreads, MINB) void batch_join_kernel(BArgs a) {
   }
   if (a.serve && tid == 0)  // other workgroups may still run: give back the cached blocks
     while (sm.ncache) free_push(a, sm.cache[--sm.ncache]);
-  const unsigned long long wg_ref = block_reduce(b_ref, sm.red);
   const unsigned long long wg_own = block_reduce(b_own, sm.red);
   if (tid == 0) {
     flush_acc(a, sm);
-    atomicAdd(&a.ctl->bytes_ref, wg_ref);
     atomicAdd(&a.ctl->bytes_own, wg_own);
     const uint64_t idle_t = __builtin_amdgcn_s_memrealtime() - t_start - busy;
     atomicAdd(&a.ctl->busy_ticks, static_cast<unsigned long long>(busy));
@@ -1575,7 +1577,7 @@ csm_status BatchRun::prepare(hipStream_t s) {
   // 2. candidate indexes of every query
   {
     std::vector<const csm_query *> cq(queries_.begin(), queries_.end());
-    csm_status st = build_cand_index_batch(ctx, g, cq, cih_, mem_, kt);
+    csm_status st = build_cand_index_batch(ctx, g, cq, cih_, mem_, kt, true);
     if (st != CSM_OK) return st;
   }
 
@@ -1595,7 +1597,7 @@ csm_status BatchRun::prepare(hipStream_t s) {
     for (uint32_t k = 0; k < nq; ++k) lvs[static_cast<size_t>(i) * CSM_MAX_QV + k] = cih_[i].level[k];
     BQuery &b = bq[i];
     std::memset(&b, 0, sizeof(b));
-    b.ci = CandIndex{q->bits, cih_[i].rank_pref, cih_[i].cn_off, cih_[i].cn_len, cih_[i].cn_adj, words};
+    b.ci = CandIndex{q->bits, cih_[i].rank_pref, cih_[i].cn_off, cih_[i].cn_len, cih_[i].cn_adj, words, cih_[i].rloc, cih_[i].loc};
     b.lv = d_lv + static_cast<size_t>(i) * CSM_MAX_QV;
     b.root_bits = q->bits + static_cast<uint64_t>(q->level[0].u) * words;
     b.cand_last = q->bits + static_cast<uint64_t>(q->level[L].u) * words;
@@ -1631,19 +1633,27 @@ csm_status BatchRun::launch(hipStream_t s) {
   const csm_graph *g = g_;
   const csm_batch_opts &o = o_;
   const uint32_t nqs = static_cast<uint32_t>(queries_.size()), words = words_;
-  TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, o.block_kb ? o.block_kb : 128u,
+  TuneParams tp{o.wgs_per_cu ? o.wgs_per_cu : 4u, 16u, o.block_kb,
                 o.ilp ? std::min(o.ilp, kIlp) : kIlp, o.max_out ? std::min(o.max_out, kMaxOut) : 1u};
   // the caller's item budget stops a query with a lower bound; without one a
   // query runs in the kernel until it is done (or the kernel's time limit)
   const uint64_t budget = o.item_budget ? o.item_budget : ~0ull;
   if (o.chunk_rows) tp.root_words = std::min<uint32_t>(kMaxRootWords, std::max<uint32_t>(1, o.chunk_rows / 64));
-  if (tp.wgs_per_cu > 8 || tp.block_kb < 32 || tp.block_kb > 4096) return set_error(ctx, CSM_E_ARG, "batch tuning out of range");
-  const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;
-  if (block_words < static_cast<uint64_t>(kThreads) * CSM_MAX_QV) return set_error(ctx, CSM_E_ARG, "block too small");
   uint64_t pool_bytes = o.pool_bytes;
   const uint64_t avail = ctx->arena.largest_free();
   if (!pool_bytes) pool_bytes = std::min<uint64_t>(avail / 2, 32ull << 30);
   pool_bytes = std::min<uint64_t>(pool_bytes, avail * 3 / 4);
+  if (!tp.block_kb) {
+    // large blocks mean few tasks per row (each task costs a record, deque
+    // operations and a descriptor staging): 512 KiB while the pool still
+    // holds kDefaultBlocksPerWg of them per workgroup, else smaller, to 128
+    tp.block_kb = 512;
+    const uint64_t want = static_cast<uint64_t>(kDefaultBlocksPerWg) * ctx->num_cus * tp.wgs_per_cu;
+    while (tp.block_kb > 128 && pool_bytes / (1024ull * tp.block_kb) < want) tp.block_kb /= 2;
+  }
+  if (tp.wgs_per_cu > 8 || tp.block_kb < 32 || tp.block_kb > 4096) return set_error(ctx, CSM_E_ARG, "batch tuning out of range");
+  const uint64_t block_words = static_cast<uint64_t>(tp.block_kb) * 1024 / 4;
+  if (block_words < static_cast<uint64_t>(kThreads) * CSM_MAX_QV) return set_error(ctx, CSM_E_ARG, "block too small");
   const uint64_t num_blocks = std::max<uint64_t>(64, pool_bytes / (4 * block_words));
   const uint32_t tasks_per_query = ceil_div(std::max<uint32_t>(words, 1), tp.root_words);
   const uint64_t nroot = static_cast<uint64_t>(nqs) * tasks_per_query;
@@ -1730,15 +1740,11 @@ csm_status BatchRun::finish(csm_result *results) {
   join_kt_->flush();
   release_graph();
   // algorithmic bytes of the persistent join: its logical reads and writes,
-  // per input row of level k the row (4k) and a rank word, prefix, list start
-  // and length per candidate list looked up (28), 4 per list entry examined
-  // and per entry read by a membership search, 4(k + 1) per row written.
-  // bytes_ref: the reference's extension pass (join.cu:99-177) over the same
-  // rows, 4k + 8 (offsets) + 12 per adjacency entry of the row's parent vertex
-  // (id + candidate word) + 4(k + 1) per row written, non-tree checks and the
-  // counted last position not charged: the work the candidate index avoids.
+  // per input row of level k the row (4k) and two 16-byte locators per
+  // candidate list looked up (32), 4 per list entry examined and per entry
+  // read by a membership search, 8 per hub-bitmap word, 4(k + 1) per row
+  // written.
   kstat_[CSM_K_BATCH].bytes = hctl.bytes_own;
-  kstat_[CSM_K_BATCH].bytes_ref = hctl.bytes_ref;
   const char *tr = std::getenv("CSM_TRACE");
   const bool trace = tr && *tr == '1';
   const uint32_t grid = grid_;
@@ -2064,8 +2070,8 @@ extern "C" csm_status csm_batch_autotune(csm_ctx *ctx, const csm_graph *g, csm_q
     for (uint32_t i = 0; i < m; ++i) v.push_back(qs[static_cast<uint64_t>(i) * n / m]);
     std::vector<csm_result> res(m);
     // (workgroups per CU, output blocks per task, block KiB, items per thread)
-    static const uint32_t grid[][4] = {{4, 1, 128, 4}, {3, 1, 128, 4}, {2, 1, 128, 4}, {4, 2, 128, 4},
-                                       {3, 2, 128, 4}, {4, 1, 64, 4},  {4, 1, 256, 4}, {4, 1, 128, 2}};
+    static const uint32_t grid[][4] = {{4, 1, 512, 4}, {3, 1, 512, 4}, {2, 1, 512, 4}, {4, 2, 512, 4},
+                                       {4, 1, 128, 4}, {4, 1, 256, 4}, {4, 1, 1024, 4}, {4, 1, 512, 2}};
     csm_status st = run_batch(ctx, g, v, b, res.data());  // warm-up: code objects, arena layout
     if (st != CSM_OK) return st;
     double best = 1e300;

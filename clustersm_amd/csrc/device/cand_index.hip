@@ -229,12 +229,46 @@ __global__ __launch_bounds__(kThreads) void cn_heavy_kernel(const uint32_t *__re
   }
 }
 
+// Rank locators (blockIdx.y = query): each candidate word beside the number
+// of set bits before it in its row.
+__global__ void rank_loc_kernel(const RankJob *jobs, uint32_t words, RankLoc *const *outs) {
+  const RankJob j = jobs[blockIdx.y];
+  RankLoc *out = outs[blockIdx.y];
+  const uint64_t total = static_cast<uint64_t>(j.nq) * words;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += 256ull * gridDim.x) {
+    const uint64_t u = i / words, b = i % words;
+    const uint64_t *pref = j.out + u * (words + 1);
+    out[i] = RankLoc{pref[b] - pref[0], j.bits[i]};
+  }
+}
+
+// List locators (blockIdx.y = slot): per candidate x of the slot's source,
+// its list start and length and x's hub index in one 16-byte entry, at the
+// same entry as cn_off / cn_len.
+__global__ __launch_bounds__(kThreads) void loc_kernel(uint32_t words, const SlotDesc *__restrict__ slots,
+                                                       const uint64_t *__restrict__ cn_off,
+                                                       const uint32_t *__restrict__ cn_len,
+                                                       const int32_t *__restrict__ hub_id, CnLoc *__restrict__ loc) {
+  const SlotDesc sd = slots[blockIdx.y];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint64_t *xrow = sd.bits + static_cast<uint64_t>(sd.src) * words;
+  const uint64_t *pref = sd.rank_pref + static_cast<uint64_t>(sd.src) * (words + 1);
+  const uint64_t pref0 = pref[0];
+  for (uint32_t b = blockIdx.x * kWaves + wave; b < words; b += gridDim.x * kWaves) {  // wave-uniform
+    const uint64_t xw = xrow[b];
+    if (xw == 0 || !((xw >> lane) & 1ull)) continue;
+    const uint32_t x = b * 64u + static_cast<uint32_t>(lane);
+    const uint64_t entry = sd.base + (pref[b] - pref0) + lanes_below(xw);
+    loc[entry] = CnLoc{cn_off[entry], cn_len[entry], hub_id ? hub_id[x] : -1};
+  }
+}
+
 uint32_t grid_for(uint64_t n) { return static_cast<uint32_t>(std::min<uint64_t>(65536, (n + 255) / 256 + 1)); }
 
 }  // namespace
 
 csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
-                                  std::vector<CandIndexHost> &out, WorkMem &mem, KernelTimer &kt) {
+                                  std::vector<CandIndexHost> &out, WorkMem &mem, KernelTimer &kt, bool locators) {
   hipStream_t s = ctx->stream;
   const uint32_t nqs = static_cast<uint32_t>(qs.size());
   if (nqs == 0) return CSM_OK;
@@ -301,6 +335,23 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
   rank_words_kernel<<<dim3(std::min<uint32_t>(1024, grid_for(rank_n / nqs + 1)), nqs), 256, 0, s>>>(d_jobs, words);
   scan_u64_exclusive(rank_all, rank_n, partials, nullptr, s);
   kt.end(CSM_K_AUX, 16ull * rank_n);
+  if (locators) {
+    uint64_t rloc_n = 0;
+    for (uint32_t i = 0; i < nqs; ++i) rloc_n += static_cast<uint64_t>(qs[i]->desc.nq) * words;
+    RankLoc *rloc_all = static_cast<RankLoc *>(get(sizeof(RankLoc) * std::max<uint64_t>(rloc_n, 1)));
+    RankLoc **d_outs = static_cast<RankLoc **>(get(sizeof(RankLoc *) * nqs));
+    if (!rloc_all || !d_outs) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (rank locators)");
+    std::vector<RankLoc *> outs(nqs);
+    for (uint64_t i = 0, o = 0; i < nqs; o += static_cast<uint64_t>(qs[i]->desc.nq) * words, ++i) {
+      outs[i] = rloc_all + o;
+      out[i].rloc = outs[i];
+    }
+    CSM_HIP_TRY(ctx, hipMemcpyAsync(d_outs, outs.data(), sizeof(RankLoc *) * nqs, hipMemcpyHostToDevice, s));
+    kt.begin(CSM_K_AUX);
+    rank_loc_kernel<<<dim3(std::min<uint32_t>(1024, grid_for(rloc_n / nqs + 1)), nqs), 256, 0, s>>>(d_jobs, words,
+                                                                                                    d_outs);
+    kt.end(CSM_K_AUX, 32ull * rloc_n);
+  }
   if (slots.empty()) return CSM_OK;
   for (uint32_t i = 0; i < nqs; ++i) {
     out[i].cn_off = cn_off;
@@ -341,6 +392,17 @@ csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::v
     cn_heavy_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(npairs, 65536)), kThreads, 0, s>>>(
         g->off, g->adj, words, d_slots, heavy, npairs, cn_off, cn_len, cn_adj);
     kt.end(CSM_K_AUX, 16ull * npairs);
+  }
+  if (locators) {
+    CnLoc *loc = static_cast<CnLoc *>(get(sizeof(CnLoc) * std::max<uint64_t>(entries, 1)));
+    if (!loc) return set_error(ctx, CSM_E_NOMEM, "arena exhausted (list locators)");
+    for (size_t s0 = 0; s0 < slots.size(); s0 += 65535) {
+      const uint32_t ns = static_cast<uint32_t>(std::min<size_t>(65535, slots.size() - s0));
+      kt.begin(CSM_K_AUX);
+      loc_kernel<<<dim3(gx, ns), kThreads, 0, s>>>(words, d_slots + s0, cn_off, cn_len, g->hub_id, loc);
+      kt.end(CSM_K_AUX, 8ull * words * ns + 32ull * cand_words * ns / slots.size());
+    }
+    for (uint32_t i = 0; i < nqs; ++i) out[i].loc = loc;
   }
   for (uint32_t i = 0; i < nqs; ++i) {
     out[i].cn_adj = cn_adj;

@@ -20,6 +20,19 @@ struct LevelJoin {
   uint64_t off_base[CSM_MAX_QV + 1];  // start of the anchor's slot in cn_off
 };
 
+// Two-load list locator (built on request): a candidate bitmap word with the
+// number of set bits before it in its row, then per (slot, candidate) the
+// list's start, length and the candidate's hub index, each one 16-byte load.
+struct alignas(16) RankLoc {
+  uint64_t pref;  // set bits of the row before this word
+  uint64_t bits;  // the word
+};
+struct alignas(16) CnLoc {
+  uint64_t start;  // in cn_adj
+  uint32_t len;
+  int32_t hub;     // the candidate's hub index, -1 for none
+};
+
 struct CandIndex {
   const uint64_t *bits;       // nq x words candidate bitmaps
   const uint64_t *rank_pref;  // nq x (words + 1): exclusive popcount prefix, global over all rows
@@ -27,6 +40,8 @@ struct CandIndex {
   const uint32_t *cn_len;     // and its length
   const uint32_t *cn_adj;
   uint32_t words;
+  const RankLoc *rloc;        // nq x words (null unless built)
+  const CnLoc *loc;           // one per cn_off entry (null unless built)
 
   // rank of x among the set bits of C(u); x must be a candidate of u
   __device__ inline uint64_t rank(uint32_t u, uint32_t x) const {
@@ -41,6 +56,13 @@ struct CandIndex {
     beg = cn_off[i];
     end = beg + cn_len[i];
   }
+  // the same list, with x's hub index, through the locators (two dependent
+  // 16-byte loads)
+  __device__ inline CnLoc locate(const LevelJoin &L, uint32_t a, uint32_t x) const {
+    const RankLoc r = rloc[static_cast<uint64_t>(L.src_u[a]) * words + (x >> 6)];
+    const uint64_t rk = r.pref + static_cast<uint64_t>(__popcll(r.bits & ((1ull << (x & 63)) - 1)));
+    return loc[L.off_base[a] + rk];
+  }
 };
 
 struct CandIndexHost {
@@ -48,6 +70,8 @@ struct CandIndexHost {
   uint64_t *cn_off = nullptr;
   uint32_t *cn_len = nullptr;
   uint32_t *cn_adj = nullptr;
+  RankLoc *rloc = nullptr;    // locators, when requested
+  CnLoc *loc = nullptr;
   uint64_t total_lists = 0;   // entries of cn_adj
   uint64_t build_bytes = 0;   // algorithmic bytes of the build
   LevelJoin level[CSM_MAX_QV];
@@ -72,7 +96,10 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
 csm_status apply_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q);
 
 // All queries' indexes with one launch per pass (one shared list array).
+// With `locators`, also the two-load locators (rloc, loc) the persistent
+// kernel reads.
 csm_status build_cand_index_batch(csm_ctx *ctx, const csm_graph *g, const std::vector<const csm_query *> &qs,
-                                  std::vector<CandIndexHost> &out, WorkMem &mem, KernelTimer &kt);
+                                  std::vector<CandIndexHost> &out, WorkMem &mem, KernelTimer &kt,
+                                  bool locators = false);
 
 }  // namespace csm

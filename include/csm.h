@@ -309,7 +309,7 @@ typedef struct csm_batch_opts {
   uint32_t chunk_rows;     /* root ids per root task (multiple of 64), 0 = default 1024 */
   uint32_t wgs_per_cu;     /* persistent workgroups per CU (1..8), 0 = default 4 */
   uint64_t pool_bytes;     /* frontier pool, 0 = auto */
-  uint32_t block_kb;       /* pool block size in KiB (32..4096), 0 = default 128 */
+  uint32_t block_kb;       /* pool block size in KiB (32..4096), 0 = default (512, smaller when the pool is small) */
   uint32_t max_out;        /* full output blocks a task holds before it yields (1..8), 0 = default 1 */
   uint32_t ilp;            /* items per thread per tile (1..4), 0 = default 4 */
   uint32_t time_limit_ms;  /* 0: none (the kernel still stops after 120 s and the plain path finishes
