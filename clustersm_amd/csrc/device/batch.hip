@@ -352,6 +352,7 @@ __global__ void batch_init_kernel(BArgs a, uint32_t nwgs) {
 
 constexpr uint32_t kIlp = 4;     // items per thread per tile, checked in lockstep
 constexpr uint64_t kHubTag = 1ull << 63;  // abeg entry of a hub anchor: tag | hub index
+constexpr uint32_t kRowLoads = 8;         // row words each thread has in flight while staging rows
 constexpr uint32_t kMaxA = 4;    // anchors whose candidate lists are kept per row
 constexpr uint32_t kMaxOut = 8;  // most full output blocks a task holds before it yields
 constexpr uint32_t kCache = 16;  // free blocks a workgroup keeps for itself
@@ -380,7 +381,7 @@ struct Smem {
   uint8_t drv[kRowBatch];
   uint8_t out_ri[kThreads * kIlp];  // a tile's survivors in output order: row in the batch
   uint32_t out_w[kThreads * kIlp];  // and the new id
-  unsigned long long red[kWaves];
+  unsigned long long red[2 * kWaves];
   uint32_t tile_n[3];               // rotating survivor counters, one per tile in flight
   uint32_t out_block;
   uint32_t rec;                     // record of the task being processed
@@ -510,6 +511,26 @@ __device__ void flush_acc(const BArgs &a, Smem &sm) {
   }
   if (sm.acc_rows) atomicAdd(&a.st[q].rows, sm.acc_rows);
   sm.acc_count = sm.acc_items = sm.acc_rows = 0;
+}
+
+// Two sums at once: one LDS round and two barriers for both (lds holds
+// 2 * kWaves values).
+__device__ inline void block_reduce2(unsigned long long &x, unsigned long long &y, unsigned long long *lds) {
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    x += __shfl_xor(x, d, kWave);
+    y += __shfl_xor(y, d, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    lds[threadIdx.x / kWave] = x;
+    lds[kWaves + threadIdx.x / kWave] = y;
+  }
+  __syncthreads();
+  x = y = 0;
+  for (int w = 0; w < kWaves; ++w) {
+    x += lds[w];
+    y += lds[kWaves + w];
+  }
+  __syncthreads();
 }
 
 template <typename T>
@@ -725,16 +746,35 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   for (; row0 < nrows; row0 += kRowBatch) {
     // the early-stop check and the batch's rows in one round trip (rows of a
     // batch that turns out stopped are read in bounds and dropped)
-    if (tid == 0) {
-      sm.stop = ld_done(a, t.q) != kRunning || aborted(a);
-      if (sm.stop && ld_done(a, t.q) == kRunning) __hip_atomic_store(&a.st[t.q].lost, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // cut short by an early stop
+    if (tid == 0) {  // both flags read at once
+      const uint32_t dn = ld_done(a, t.q);
+      const bool ab = aborted(a);
+      sm.stop = dn != kRunning || ab;
+      if (ab && dn == kRunning) __hip_atomic_store(&a.st[t.q].lost, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // cut short by an early stop
     }
     const uint32_t nb = min(kRowBatch, nrows - row0);
-    for (uint32_t i = tid; i < nb * k; i += kThreads)
-      s_rows[i] = k == 1    ? sm.roots[row0 + i]
-                  : stolen ? __hip_atomic_load(&grows[static_cast<uint64_t>(row0) * k + i], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT)
-                           : grows[static_cast<uint64_t>(row0) * k + i];
+    const uint32_t nwords = nb * k;
+    if (k == 1) {
+      for (uint32_t i = tid; i < nwords; i += kThreads) s_rows[i] = sm.roots[row0 + i];
+    } else {
+      // kRowLoads loads per thread issued before any is waited for
+      const uint32_t *src = grows + static_cast<uint64_t>(row0) * k;
+      for (uint32_t i0 = 0; i0 < nwords; i0 += kRowLoads * kThreads) {
+        uint32_t v[kRowLoads];
+#pragma unroll
+        for (uint32_t j = 0; j < kRowLoads; ++j) {
+          const uint32_t i = i0 + j * kThreads + tid;
+          v[j] = i >= nwords ? 0u
+                 : stolen   ? __hip_atomic_load(&src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : src[i];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kRowLoads; ++j) {
+          const uint32_t i = i0 + j * kThreads + tid;
+          if (i < nwords) s_rows[i] = v[j];
+        }
+      }
+    }
     __syncthreads();
     if (sm.stop) break;
     uint64_t tq = 0;  // staging sub-phases (thread 0): rows, list lookups, drivers and scan
@@ -877,6 +917,34 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       for (uint32_t j = 0; j < kIlp; ++j) w[j] = ok[j] ? Q.ci.cn_adj[sm.dbeg[ri[j]] + (its[j] - sm.pre[ri[j]])] : 0u;
 #pragma unroll
       for (uint32_t j = 0; j < kIlp; ++j) items_seen += ok[j] ? 1 : 0;
+      // Two anchors (uniform): the other anchor's list is searched with its
+      // first two steps' entries (the middle, and the middles of both halves)
+      // loaded beside the item, before the item is compared with anything, so
+      // they arrive with it and two dependent rounds disappear.
+      uint64_t plo[kIlp];
+      uint32_t pn[kIlp], p1[kIlp], p2a[kIlp], p2b[kIlp];
+      bool phub[kIlp];
+      if (nanchor == 2) {
+#pragma unroll
+        for (uint32_t j = 0; j < kIlp; ++j) {
+          const uint32_t xo = 1u - sm.drv[ri[j]];
+          const uint64_t b = sm.abeg[xo][ri[j]];
+          phub[j] = ok[j] && (b & kHubTag) != 0;
+          plo[j] = phub[j] ? (b & ~kHubTag) : b;  // hub index, or list start
+          pn[j] = ok[j] && !phub[j] ? sm.alen[xo][ri[j]] : 0u;
+          p1[j] = p2a[j] = p2b[j] = 0;
+          if (pn[j] > 1) {
+            const uint32_t h = pn[j] >> 1, h2 = (pn[j] - h) >> 1;
+            p1[j] = Q.ci.cn_adj[plo[j] + h];
+            probe_loads += 1;
+            if (h2) {
+              p2a[j] = Q.ci.cn_adj[plo[j] + h2];
+              p2b[j] = Q.ci.cn_adj[plo[j] + h + h2];
+              probe_loads += 2;
+            }
+          }
+        }
+      }
       // injectivity against rows in LDS
       for (uint32_t x = 0; x < sm.lv.ninj; ++x) {
         const uint32_t col = sm.lv.inj[x];
@@ -884,9 +952,36 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
         for (uint32_t j = 0; j < kIlp; ++j)
           if (ok[j]) ok[j] = s_rows[ri[j] * k + col] != w[j];
       }
+      if (nanchor == 2) {
+        bool f[kIlp];
+#pragma unroll
+        for (uint32_t j = 0; j < kIlp; ++j) {
+          f[j] = false;
+          if (phub[j]) {
+            f[j] = (a.probe.hub_bits[plo[j] * a.words + (w[j] >> 6)] >> (w[j] & 63)) & 1ull;
+            hub_probes += 1;
+            pn[j] = 0;
+          } else if (pn[j] > 1) {  // the two prefetched steps
+            const uint32_t h = pn[j] >> 1;
+            const bool right = p1[j] <= w[j];
+            plo[j] += right ? h : 0;
+            pn[j] -= h;
+            if (pn[j] > 1) {
+              const uint32_t h2 = pn[j] >> 1;
+              if ((right ? p2b[j] : p2a[j]) <= w[j]) plo[j] += h2;
+              pn[j] -= h2;
+            }
+          }
+        }
+        bool g[kIlp];
+        lockstep_find<kIlp>(Q.ci.cn_adj, plo, pn, w, g, probe_loads);
+#pragma unroll
+        for (uint32_t j = 0; j < kIlp; ++j)
+          if (ok[j]) ok[j] = phub[j] ? f[j] : g[j];
+      }
       // adjacency to the other anchors: w is in C(u_k), so w ~ r[a] iff w is
       // in the anchor's candidate list; binary searches run in lockstep
-      for (uint32_t x = 0; x < nanchor; ++x) {
+      for (uint32_t x = 0; x < (nanchor == 2 ? 0u : nanchor); ++x) {
         if (x < kMaxA) {
           // an anchor with a dense adjacency bitmap (abeg holds kHubTag | its
           // hub index): one word tells w ~ r[x], since w is a candidate of
@@ -1044,9 +1139,9 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
       // the row batch's count is added at once, so the query's total passes
       // its cap within one row batch of the embedding that crosses it and
       // every workgroup stops its tasks at their next row batch
-      const unsigned long long part = block_reduce(contrib, sm.red);
-      const unsigned long long its = block_reduce(items_seen, sm.red);
       b_own += 4ull * items_seen;
+      unsigned long long part = contrib, its = items_seen;
+      block_reduce2(part, its, sm.red);
       if (tid == 0) {
         sm.acc_count += part;
         sm.acc_items += its;
@@ -1074,9 +1169,9 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   // to the shared free ring (free_push), each an L2 write-back after these
   // stores; the taker acquires (steal, free_pop) before reading.
   __builtin_amdgcn_s_waitcnt(0);
-  const unsigned long long tot_contrib = block_reduce(contrib, sm.red);
-  const unsigned long long tot_items = block_reduce(items_seen, sm.red);
   b_own += 4ull * items_seen + 4ull * probe_loads + 8ull * hub_probes;  // list entries, searches, bitmap words
+  unsigned long long tot_contrib = contrib, tot_items = items_seen;
+  block_reduce2(tot_contrib, tot_items, sm.red);
   b_own += 4ull * kout * rows_written;
   if (tid == 0) {
     const bool live = ld_done(a, t.q) == kRunning && !aborted(a);
