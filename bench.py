@@ -279,9 +279,23 @@ def c2_section(ctx, limits_ms):
         print(f"# C2 {name}: {'exact' if done else 'lower bound'} {r.count} in {dt:.2f}s", file=sys.stderr,
               flush=True)
         tot = kernel_totals([[r]])
-        out.append({"name": name, "count": r.count, "exact": done, "seconds": round(dt, 3),
-                    "time_limit_ms": lim, "embeddings_per_s": round(r.count / dt, 1), "items": r.items,
-                    "roofline": roofline_from(tot, "count_kernel_rmat16_plain.md")})
+        entry = {"name": name, "count": r.count, "exact": done, "seconds": round(dt, 3),
+                 "time_limit_ms": lim, "embeddings_per_s": round(r.count / dt, 1), "items": r.items,
+                 "roofline": roofline_from(tot, "count_kernel_rmat16_plain.md")}
+        if name == "cycles+3":
+            # the same first 5 s of the search with the count level's lists
+            # copied into LDS (lds_lists option), beside the default
+            rate = {}
+            for lds in (False, True):
+                t = time.perf_counter()
+                rs, _ = q.match(cap=0, time_limit_ms=5000, lds_lists=lds)
+                rate[lds] = rs.items / (time.perf_counter() - t)
+            entry["lds_lists_sample"] = {"seconds": 5, "default_items_per_s": round(rate[False], 1),
+                                         "lds_lists_items_per_s": round(rate[True], 1),
+                                         "ratio": round(rate[True] / rate[False], 3)}
+            print(f"# C2 {name}: lds_lists {rate[True]:.3g} items/s, default {rate[False]:.3g}", file=sys.stderr,
+                  flush=True)
+        out.append(entry)
         q.free()
     g.free()
     return {"workload": "plain path, R-MAT scale 16 (2^20 samples, 16 Zipf(1.0) labels), two 8-vertex random-walk "

@@ -101,7 +101,7 @@ class MatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("refine", C.c_uint32), ("reuse_filter", C.c_uint32),
                 ("list", C.POINTER(C.c_uint32)), ("list_capacity", C.c_uint64),
                 ("chunk_items", C.c_uint64), ("time_limit_ms", C.c_uint32), ("reorder", C.c_uint32),
-                ("item_budget", C.c_uint64), ("symmetry", C.c_uint32), ("reserved", C.c_uint32)]
+                ("item_budget", C.c_uint64), ("symmetry", C.c_uint32), ("lds_lists", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -465,17 +465,20 @@ class Query:
                              "csm_query_reorder")
 
     def match(self, cap=0, want_list=False, list_capacity=None, refine=True, reuse_filter=False, chunk_items=0,
-              time_limit_ms=0, reorder=False, item_budget=0, symmetry=False):
+              time_limit_ms=0, reorder=False, item_budget=0, symmetry=False, lds_lists=False):
         """Returns (Result, rows or None); rows are sorted, in query-id order.
         With time_limit_ms (item_budget), Result.status == E_TIMEOUT (E_BUDGET)
         marks a stopped search whose count is a lower bound.
-        reorder: candidate-aware matching order after the filter (same counts)."""
+        reorder: candidate-aware matching order after the filter (same counts).
+        lds_lists: the count level searches LDS copies of its rows' short
+        candidate lists (same counts)."""
         o = MatchOpts()
         o.cap, o.refine, o.reuse_filter, o.chunk_items = cap, 1 if refine else 0, 1 if reuse_filter else 0, chunk_items
         o.time_limit_ms = time_limit_ms
         o.item_budget = item_budget
         o.symmetry = 1 if symmetry else 0
         o.reorder = 1 if reorder else 0
+        o.lds_lists = 1 if lds_lists else 0
         rows = None
         if want_list:
             lc = list_capacity if list_capacity is not None else (cap if cap else 1 << 20)

@@ -17,9 +17,10 @@
 //   expand<MARK>        test each item, 64-bit survivor mask per wave
 //   scan                tile survivor counts -> output offsets
 //   expand<MATERIALIZE> write surviving rows (prefix + new vertex), item order
-//   expand<COUNT>       last position: count survivors
-//   expand<FUSED>       last position without non-tree edges, counted from the
-//                       position before it: each surviving item adds the
+//   count<false>        last position: count survivors, four tiles per
+//                       workgroup and four items per thread in lockstep
+//   count<true>         last position without non-tree edges, counted from
+//                       the position before it: each surviving item adds the
 //                       length of N(x) ∩ C(u_last) for its parent vertex x
 //                       minus the row's own vertices in that list
 //
@@ -75,9 +76,10 @@ struct ExpandArgs {
   uint32_t fused_rows;
   uint32_t last_inj_k;      // the last position's label equals position k's
   LevelSym sym;             // symmetry-breaking conditions at position k (MARK / COUNT)
+  unsigned long long *extra;  // count kernel: bytes read beyond rows, prefix, drivers and list entries
 };
 
-enum { kMark = 0, kCount = 1, kMaterialize = 2, kFused = 3 };
+enum { kMark = 0, kMaterialize = 2 };
 
 // v in the sorted list cn[beg .. beg + len)
 __device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint64_t len, uint32_t v) {
@@ -120,7 +122,6 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   __shared__ uint64_t sd[kTileRows + 1];
   __shared__ uint64_t s_base[kTileRows];
   __shared__ uint32_t s_wcnt[kTileWaves];
-  __shared__ unsigned long long s_sum[kTileWaves];
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kTile;
   const uint64_t rlo = a.row_a + a.tile_row[blockIdx.x];
@@ -147,8 +148,6 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   bool ok = false;
   uint32_t w = 0;
   const uint32_t *row = nullptr;
-  unsigned long long contrib = 0;
-  uint64_t rsel = 0;  // the item's row: rlo + rsel (staged) or rsel (wide)
   if (valid) {
     const uint64_t gi = a.item_base + item;
     uint32_t d;
@@ -158,13 +157,11 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
         const uint32_t mid = (lo + hi) >> 1;
         if (sd[mid] <= gi) lo = mid; else hi = mid;
       }
-      rsel = lo;
       row = s_rows + lo * k;
       d = a.drv[rlo + lo];
       w = a.ci.cn_adj[s_base[lo] + (gi - sd[lo])];
     } else {
       const uint64_t r = upper_row(a.dscan, rlo, rend, gi);
-      rsel = r;
       row = a.rows + r * k;
       d = a.drv[r];
       uint64_t beg, end;
@@ -180,33 +177,7 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
       for (uint32_t t = 0; ok && t < a.sym.nlt; ++t) ok = w < row[a.sym.lt[t]];
       for (uint32_t t = 0; ok && t < a.lv.nanchor; ++t)
         if (t != d) ok = a.probe.has(row[a.lv.anchor[t]], w);
-      if (MODE == kFused && ok) {
-        // the last position's candidates adjacent to its parent x, minus the
-        // row's vertices among them (w in C(u_last) and w ~ x iff w is in x's list)
-        const uint32_t pk = a.lv_last.parent;
-        if (a.fused_rows) {  // x = row[pk]: per-row constant from the row pass
-          const uint64_t r = (wide ? 0 : rlo) + rsel;
-          long long c = static_cast<long long>(a.fbase[r]);
-          if (a.last_inj_k && w != row[pk] && in_list(a.ci.cn_adj, a.fbeg[r], a.flen[r], w)) --c;
-          contrib = static_cast<unsigned long long>(c);
-        } else {  // x = w
-          uint64_t lb, le;
-          a.ci.list(a.lv_last, 0, w, lb, le);
-          long long c = static_cast<long long>(le - lb);
-          for (uint32_t t = 0; t < a.lv_last.ninj; ++t) {
-            const uint32_t j = a.lv_last.inj[t];
-            if (j == k) continue;
-            if (in_list(a.ci.cn_adj, lb, le - lb, row[j])) --c;
-          }
-          contrib = static_cast<unsigned long long>(c);
-        }
-      }
     }
-  }
-  if (MODE == kFused) {  // per-tile partial; one reduction kernel sums them
-    const unsigned long long tot = block_sum(contrib, s_sum);
-    if (tid == 0) a.tile_cnt[blockIdx.x] = tot;
-    return;
   }
   const uint64_t m = ballot(ok);
   if (MODE == kMaterialize) {
@@ -233,6 +204,490 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- count kernel
+// The count level's items (COUNT; FUSED: counted from the position before the
+// last), four 256-item tiles per workgroup and four items per thread, checked
+// in lockstep so that their dependent loads overlap.  Besides the driver
+// list, each row looks up the candidate list of every other kept anchor: w is
+// a candidate of u_k already, so w ~ x iff w is in N(x) ∩ C(u_k), which is
+// shorter than N(x).  Lists that many of a row's items search are copied into
+// LDS once per workgroup and searched there: the row's run of driver entries
+// is intersected with the LDS copies of its other sorted lists.
+constexpr int kCIlp = 4;
+constexpr uint32_t kCMaxA = 3;          // anchors whose candidate lists are kept per row (more: adjacency probes)
+constexpr uint32_t kCListWords = 1536;  // LDS words for copied lists
+constexpr uint64_t kCStageMin = 16;     // items a row needs in the workgroup before its lists are copied
+constexpr uint64_t kCStageRatio = 8;    // and a copied list has at most this many entries per such item
+constexpr uint64_t kCTagHub = 1ull << 63;  // list slot of a hub anchor: tag | hub index
+constexpr uint64_t kCTagLds = 1ull << 62;  // list slot copied into LDS: tag | list start
+
+// Membership of v[j] in sorted lists, all searches advancing together; for j
+// with in_lds[j], lo[j] is an offset into the LDS copies s, else a position
+// in cn.  n[j] = 0: not found.  `loads` gains the global entries read.
+template <uint32_t N>
+__device__ __forceinline__ void find_mixed(const uint32_t *cn, const uint32_t *s, uint64_t (&lo)[N], uint32_t (&n)[N],
+                                           const uint32_t (&v)[N], const bool (&in_lds)[N], bool (&found)[N],
+                                           unsigned long long &loads) {
+  bool has[N];
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) has[j] = n[j] != 0;
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) any |= n[j] > 1;
+    if (!any) break;
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+      if (n[j] > 1) {
+        const uint32_t half = n[j] >> 1;
+        const uint32_t e = in_lds[j] ? s[static_cast<uint32_t>(lo[j]) + half] : cn[lo[j] + half];
+        if (e <= v[j]) lo[j] += half;
+        n[j] -= half;
+        loads += in_lds[j] ? 0 : 1;
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) {
+    const uint32_t e = !has[j] ? 0u : in_lds[j] ? s[static_cast<uint32_t>(lo[j])] : cn[lo[j]];
+    found[j] = has[j] && e == v[j];
+    loads += has[j] && !in_lds[j] ? 1 : 0;
+  }
+}
+
+__device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t *lds, uint32_t &total) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  uint32_t x = v;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, kWave);
+    if (lane >= d) x += y;
+  }
+  if (lane == kWave - 1) lds[wave] = x;
+  __syncthreads();
+  uint32_t off = 0, all = 0;
+  for (int w = 0; w < kTileWaves; ++w) {
+    if (w < wave) off += lds[w];
+    all += lds[w];
+  }
+  __syncthreads();
+  total = all;
+  return off + x - v;
+}
+
+// v in the sorted list cn[beg .. beg + len); `loads` gains the entries read.
+__device__ inline bool in_list_n(const uint32_t *cn, uint64_t beg, uint64_t len, uint32_t v, unsigned long long &loads) {
+  while (len > 1) {
+    const uint64_t half = len >> 1;
+    if (cn[beg + half] <= v) beg += half;
+    len -= half;
+    ++loads;
+  }
+  loads += len == 1 ? 1 : 0;
+  return len == 1 && cn[beg] == v;
+}
+
+// One item of a workgroup whose rows are not staged (they span more than
+// kTileRows rows): rows, lists and adjacency from global memory.  `bytes`
+// gains what it reads beyond its list entry.
+template <bool FUSED>
+__device__ unsigned long long count_item_wide(const ExpandArgs &a, uint64_t rlo, uint64_t rend, uint64_t gi,
+                                              unsigned long long &bytes) {
+  const uint32_t k = a.k;
+  uint64_t lo = rlo, hi = rend;  // dscan[lo] <= gi < dscan[hi]
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (a.dscan[mid] <= gi) lo = mid; else hi = mid;
+    bytes += 8;
+  }
+  const uint64_t r = lo;
+  const uint32_t *row = a.rows + r * k;
+  const uint32_t d = a.drv[r];
+  uint64_t beg, end;
+  a.ci.list(a.lv, d, row[a.lv.anchor[d]], beg, end);
+  const uint32_t w = a.ci.cn_adj[beg + (gi - a.dscan[r])];
+  bytes += 4ull * k + 8 + 1 + 28;  // row, its prefix entry, driver, list lookup
+  unsigned long long loads = 0;
+  bool ok = true;
+  for (uint32_t t = 0; ok && t < a.lv.ninj; ++t) ok = row[a.lv.inj[t]] != w;
+  for (uint32_t t = 0; ok && t < a.sym.ngt; ++t) ok = w > row[a.sym.gt[t]];
+  for (uint32_t t = 0; ok && t < a.sym.nlt; ++t) ok = w < row[a.sym.lt[t]];
+  for (uint32_t t = 0; ok && t < a.lv.nanchor; ++t) {
+    if (t == d) continue;
+    const uint32_t x = row[a.lv.anchor[t]];
+    const int32_t h = a.probe.hub_id[x];
+    if (h >= 0) {
+      ok = (a.probe.hub_bits[static_cast<uint64_t>(h) * a.probe.words + (w >> 6)] >> (w & 63)) & 1ull;
+      bytes += 12;
+    } else {
+      ok = in_list_n(a.probe.adj, a.probe.off[x], a.probe.off[x + 1] - a.probe.off[x], w, loads);
+      bytes += 12;
+    }
+  }
+  unsigned long long c = ok ? 1 : 0;
+  if (FUSED && ok) {
+    const uint32_t pk = a.lv_last.parent;
+    if (a.fused_rows) {
+      c = a.fbase[r];
+      bytes += 4;
+      if (a.last_inj_k && w != row[pk]) {
+        bytes += 12;
+        if (in_list_n(a.ci.cn_adj, a.fbeg[r], a.flen[r], w, loads)) --c;
+      }
+    } else {
+      uint64_t lb, le;
+      a.ci.list(a.lv_last, 0, w, lb, le);
+      bytes += 28;
+      c = le - lb;
+      for (uint32_t t = 0; t < a.lv_last.ninj; ++t) {
+        const uint32_t j = a.lv_last.inj[t];
+        if (j != k && row[j] != w && in_list_n(a.ci.cn_adj, lb, le - lb, row[j], loads)) --c;
+      }
+    }
+  }
+  bytes += 4 * loads;
+  return c;
+}
+
+// Grid: ceil(ntiles / kCIlp) workgroups; tile_cnt[blockIdx.x] = the
+// workgroup's count; *a.extra gains the bytes of list lookups, searches,
+// hub probes and LDS copies.
+template <bool FUSED, bool LISTS>
+__global__ __launch_bounds__(kTile) void count_kernel(ExpandArgs a, uint32_t ntiles) {
+  extern __shared__ uint32_t s_rows[];              // kTileRows x k ids
+  __shared__ uint64_t sd[kTileRows + 1];            // driver-length prefix of the rows
+  __shared__ uint64_t s_base[kTileRows];            // driver list start
+  __shared__ uint64_t s_lb[kCMaxA + 1][kTileRows];  // kept anchors' lists (slot kCMaxA: the fused list): start | tag
+  __shared__ uint32_t s_ll[kCMaxA + 1][kTileRows];  // and length
+  __shared__ uint16_t s_lo[kCMaxA + 1][kTileRows];  // offset of a copied list in s_list
+  __shared__ uint32_t s_fbase[kTileRows];
+  __shared__ uint8_t s_drv[kTileRows];
+  __shared__ uint32_t s_spre[kTileRows];            // words copied before each row
+  __shared__ uint32_t s_list[kCListWords];
+  __shared__ uint32_t s_scan[kTileWaves];
+  __shared__ uint32_t s_tot;
+  __shared__ unsigned long long s_sum[kTileWaves];
+  const int tid = threadIdx.x;
+  const uint32_t g0 = blockIdx.x * kCIlp, g1 = min(g0 + kCIlp, ntiles);
+  const uint64_t first = static_cast<uint64_t>(g0) * kTile;                  // items [first, last) of the chunk
+  const uint64_t last = min(static_cast<uint64_t>(g1) * kTile, a.items);
+  const uint64_t rlo = a.row_a + a.tile_row[g0];
+  const uint64_t rend = min(a.row_a + a.tile_row[g1] + 1, a.row_b);  // exclusive
+  const uint64_t span = rend - rlo;
+  const bool wide = span > static_cast<uint64_t>(kTileRows);
+  const uint32_t nr = wide ? 0u : static_cast<uint32_t>(span);
+  const uint32_t k = a.k;
+  const uint32_t nanchor = a.lv.nanchor, nkept = min(nanchor, kCMaxA);
+  const bool frows = FUSED && a.fused_rows != 0;
+  const bool flist = frows && a.last_inj_k != 0;  // items search the row's fused list
+  unsigned long long contrib = 0, loads = 0, bytes = 0;
+  if (wide) {
+#pragma unroll
+    for (int j = 0; j < kCIlp; ++j) {
+      const uint64_t item = first + static_cast<uint64_t>(j) * kTile + tid;
+      if (item < last) contrib += count_item_wide<FUSED>(a, rlo, rend, a.item_base + item, bytes);
+    }
+  } else {
+    // stage: prefix, rows (coalesced), drivers, the fused count's per-row terms
+    for (uint32_t i = tid; i <= nr; i += kTile) sd[i] = a.dscan[rlo + i];
+    for (uint32_t i = tid; i < nr * k; i += kTile) s_rows[i] = a.rows[rlo * k + i];
+    for (uint32_t i = tid; i < nr; i += kTile) {
+      s_drv[i] = a.drv[rlo + i];
+      if (frows) s_fbase[i] = a.fbase[rlo + i];
+      if (LISTS && flist) {
+        s_lb[kCMaxA][i] = a.fbeg[rlo + i];
+        s_ll[kCMaxA][i] = a.flen[rlo + i];
+      }
+    }
+    __syncthreads();
+    if constexpr (LISTS) {
+      // list lookups, one (row, anchor) pair per thread, all in flight at once;
+      // a hub anchor other than the driver is tested on its adjacency bitmap
+      for (uint32_t p = tid; p < nr * nkept; p += kTile) {
+        const uint32_t r = p / nkept, t = p - r * nkept;
+        const uint32_t x = s_rows[r * k + a.lv.anchor[t]];
+        const int32_t h = t == s_drv[r] ? -1 : a.probe.hub_id[x];
+        if (h >= 0) {
+          s_lb[t][r] = kCTagHub | static_cast<uint64_t>(h);
+        } else {
+          uint64_t beg, end;
+          a.ci.list(a.lv, t, x, beg, end);
+          if (t == s_drv[r]) s_base[r] = beg;
+          s_lb[t][r] = beg;
+          s_ll[t][r] = static_cast<uint32_t>(end - beg);
+        }
+      }
+      for (uint32_t r = tid; r < nr; r += kTile) {  // a driver beyond the kept anchors
+        const uint32_t d = s_drv[r];
+        if (d >= nkept) {
+          uint64_t beg, end;
+          a.ci.list(a.lv, d, s_rows[r * k + a.lv.anchor[d]], beg, end);
+          s_base[r] = beg;
+        }
+      }
+      // beyond the rows, prefix and drivers the host counts: a list lookup
+      // (rank word, prefix, start, length) or hub index per kept anchor, and
+      // the fused count's per-row terms
+      if (tid == 0) bytes += nr * (32ull * nkept + (frows ? 4 : 0) + (flist ? 12 : 0));
+      __syncthreads();
+      // copies: a row with at least kCStageMin items here copies its short
+      // non-hub lists (slot order) at its place in the prefix of the words the
+      // rows need; rows past the buffer's end keep searching global memory
+      uint32_t need = 0;
+      uint64_t most = 0;
+      if (static_cast<uint32_t>(tid) < nr) {
+        const uint64_t lo = max(sd[tid], a.item_base + first), hi = min(sd[tid + 1], a.item_base + last);
+        const uint64_t cnt = hi > lo ? hi - lo : 0;
+        if (cnt >= kCStageMin) {
+          most = min<uint64_t>(kCListWords, kCStageRatio * cnt);
+          for (uint32_t t = 0; t < nkept; ++t)
+            if (t != s_drv[tid] && !(s_lb[t][tid] & kCTagHub) && s_ll[t][tid] <= most) need += s_ll[t][tid];
+          if (flist && s_ll[kCMaxA][tid] <= most) need += s_ll[kCMaxA][tid];
+        }
+      }
+      uint32_t tot = 0;
+      if (__syncthreads_or(need != 0)) {
+        if (tid == 0) s_tot = 0;
+        uint32_t all;
+        const uint32_t off = block_excl_scan_u32(need, s_scan, all);
+        if (static_cast<uint32_t>(tid) < nr) s_spre[tid] = off;
+        if (need && off + need <= kCListWords) {
+          uint32_t o = off;
+          for (uint32_t t = 0; t < nkept; ++t) {
+            if (t != s_drv[tid] && !(s_lb[t][tid] & kCTagHub) && s_ll[t][tid] <= most) {
+              s_lo[t][tid] = static_cast<uint16_t>(o);
+              s_lb[t][tid] |= kCTagLds;
+              o += s_ll[t][tid];
+            }
+          }
+          if (flist && s_ll[kCMaxA][tid] <= most) {
+            s_lo[kCMaxA][tid] = static_cast<uint16_t>(o);
+            s_lb[kCMaxA][tid] |= kCTagLds;
+          }
+          atomicMax(&s_tot, off + need);
+        }
+        __syncthreads();
+        tot = s_tot;
+      }
+      // word i: the last row whose prefix is <= i (a row that copies nothing has
+      // the next row's prefix), then the list of that row holding it
+      for (uint32_t i0 = 0; i0 < tot; i0 += 4 * kTile) {
+        uint32_t v[4];
+  #pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t i = i0 + j * kTile + tid;
+          v[j] = 0;
+          if (i < tot) {
+            uint32_t r = 0;
+  #pragma unroll
+            for (uint32_t step = 256; step > 0; step >>= 1)
+              if (r + step < nr && s_spre[r + step] <= i) r += step;
+            uint32_t il = i - s_spre[r];
+            uint64_t src = ~0ull;
+            for (uint32_t t = 0; t < nkept && src == ~0ull; ++t) {
+              const uint64_t b = s_lb[t][r];
+              if (t != s_drv[r] && (b & kCTagLds)) {
+                if (il < s_ll[t][r]) src = (b & ~kCTagLds) + il;
+                else il -= s_ll[t][r];
+              }
+            }
+            if (src == ~0ull) src = (s_lb[kCMaxA][r] & ~kCTagLds) + il;  // the fused list comes last
+            v[j] = a.ci.cn_adj[src];
+          }
+        }
+  #pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t i = i0 + j * kTile + tid;
+          if (i < tot) s_list[i] = v[j];
+        }
+      }
+      bytes += tid == 0 ? 4ull * tot : 0;
+      if (tot) __syncthreads();
+    } else {  // the driver's list only; other anchors are probed on their adjacency
+      for (uint32_t r = tid; r < nr; r += kTile) {
+        const uint32_t d = s_drv[r];
+        uint64_t beg, end;
+        a.ci.list(a.lv, d, s_rows[r * k + a.lv.anchor[d]], beg, end);
+        s_base[r] = beg;
+      }
+      if (tid == 0) bytes += nr * (28ull + (frows ? 4 : 0));
+      __syncthreads();
+    }
+
+    // items: row by binary search over the prefix, then the new vertex
+    bool act[kCIlp], ok[kCIlp];
+    uint64_t gi[kCIlp];
+    uint32_t r[kCIlp], w[kCIlp];
+#pragma unroll
+    for (int j = 0; j < kCIlp; ++j) {
+      const uint64_t item = first + static_cast<uint64_t>(j) * kTile + tid;
+      act[j] = item < last;
+      gi[j] = a.item_base + item;
+      r[j] = 0;
+    }
+#pragma unroll
+    for (uint32_t step = 256; step > 0; step >>= 1) {
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) {
+        const uint32_t c = r[j] + step;
+        if (act[j] && c < nr && sd[c] <= gi[j]) r[j] = c;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kCIlp; ++j) w[j] = act[j] ? a.ci.cn_adj[s_base[r[j]] + (gi[j] - sd[r[j]])] : 0u;
+#pragma unroll
+    for (int j = 0; j < kCIlp; ++j) ok[j] = act[j];
+    for (uint32_t x = 0; x < a.lv.ninj; ++x) {
+      const uint32_t col = a.lv.inj[x];
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) ok[j] = ok[j] && s_rows[r[j] * k + col] != w[j];
+    }
+    for (uint32_t x = 0; x < a.sym.ngt; ++x) {
+      const uint32_t col = a.sym.gt[x];
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) ok[j] = ok[j] && w[j] > s_rows[r[j] * k + col];
+    }
+    for (uint32_t x = 0; x < a.sym.nlt; ++x) {
+      const uint32_t col = a.sym.lt[x];
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) ok[j] = ok[j] && w[j] < s_rows[r[j] * k + col];
+    }
+    if constexpr (LISTS) {
+      // the other anchors: hub bitmap word, or the anchor's list (LDS copy or
+      // global), searched in lockstep
+      for (uint32_t t = 0; t < nanchor; ++t) {
+        if (t < nkept) {
+          uint64_t lo[kCIlp];
+          uint32_t n[kCIlp];
+          bool in_lds[kCIlp], hub[kCIlp], hbit[kCIlp], f[kCIlp];
+  #pragma unroll
+          for (int j = 0; j < kCIlp; ++j) {
+            const bool need_j = ok[j] && t != s_drv[r[j]];
+            const uint64_t b = need_j ? s_lb[t][r[j]] : 0ull;
+            hub[j] = need_j && (b & kCTagHub) != 0;
+            in_lds[j] = need_j && (b & kCTagLds) != 0;
+            lo[j] = in_lds[j] ? s_lo[t][r[j]] : (b & ~kCTagHub);
+            n[j] = need_j && !hub[j] ? s_ll[t][r[j]] : 0u;
+            hbit[j] = false;
+            if (hub[j]) {
+              hbit[j] = (a.probe.hub_bits[lo[j] * a.probe.words + (w[j] >> 6)] >> (w[j] & 63)) & 1ull;
+              bytes += 8;
+            }
+          }
+          find_mixed<kCIlp>(a.ci.cn_adj, s_list, lo, n, w, in_lds, f, loads);
+  #pragma unroll
+          for (int j = 0; j < kCIlp; ++j)
+            if (ok[j] && t != s_drv[r[j]]) ok[j] = hub[j] ? hbit[j] : f[j];
+        } else {
+  #pragma unroll
+          for (int j = 0; j < kCIlp; ++j)
+            if (ok[j] && t != s_drv[r[j]]) ok[j] = a.probe.has(s_rows[r[j] * k + a.lv.anchor[t]], w[j]);
+        }
+      }
+    } else {
+      // the other anchors x: hub bitmap word, else a search of N(x); the
+      // hub index and both adjacency offsets are loaded together
+      for (uint32_t t = 0; t < nanchor; ++t) {
+        uint64_t lo[kCIlp];
+        uint32_t n[kCIlp], x[kCIlp], o1[kCIlp];
+        int32_t h[kCIlp];
+        bool need[kCIlp], hbit[kCIlp], f[kCIlp], in_lds[kCIlp];
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j) {
+          need[j] = ok[j] && t != s_drv[r[j]];
+          x[j] = need[j] ? s_rows[r[j] * k + a.lv.anchor[t]] : 0u;
+          h[j] = need[j] ? a.probe.hub_id[x[j]] : -1;
+          lo[j] = need[j] ? a.probe.off[x[j]] : 0u;
+          o1[j] = need[j] ? a.probe.off[x[j] + 1] : 0u;
+          in_lds[j] = false;
+        }
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j) {
+          hbit[j] = false;
+          n[j] = need[j] && h[j] < 0 ? o1[j] - static_cast<uint32_t>(lo[j]) : 0u;
+          if (need[j] && h[j] >= 0) {
+            hbit[j] = (a.probe.hub_bits[static_cast<uint64_t>(h[j]) * a.probe.words + (w[j] >> 6)] >> (w[j] & 63)) & 1ull;
+            bytes += 8;
+          }
+          bytes += need[j] ? 12 : 0;
+        }
+        find_mixed<kCIlp>(a.probe.adj, s_rows, lo, n, w, in_lds, f, loads);
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j)
+          if (need[j]) ok[j] = h[j] >= 0 ? hbit[j] : f[j];
+      }
+    }
+    if (!FUSED) {
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) contrib += ok[j] ? 1 : 0;
+    } else if (frows) {
+      // x = the row's parent of the last position: the row's count, minus w
+      // when w is in that list
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) contrib += ok[j] ? s_fbase[r[j]] : 0u;
+      if (flist) {
+        const uint32_t pk = a.lv_last.parent;
+        uint64_t lo[kCIlp];
+        uint32_t n[kCIlp];
+        bool in_lds[kCIlp], f[kCIlp];
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j) {
+          const bool need_j = ok[j] && w[j] != s_rows[r[j] * k + pk];
+          if constexpr (LISTS) {
+            const uint64_t b = need_j ? s_lb[kCMaxA][r[j]] : 0ull;
+            in_lds[j] = need_j && (b & kCTagLds) != 0;
+            lo[j] = in_lds[j] ? s_lo[kCMaxA][r[j]] : b;
+            n[j] = need_j ? s_ll[kCMaxA][r[j]] : 0u;
+          } else {
+            in_lds[j] = false;
+            lo[j] = need_j ? a.fbeg[rlo + r[j]] : 0ull;
+            n[j] = need_j ? a.flen[rlo + r[j]] : 0u;
+            bytes += need_j ? 12 : 0;
+          }
+        }
+        find_mixed<kCIlp>(a.ci.cn_adj, LISTS ? s_list : s_rows, lo, n, w, in_lds, f, loads);
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j) contrib -= f[j] ? 1u : 0u;
+      }
+    } else {
+      // x = w: w's list of the last position, minus the row's own members
+      uint64_t lb[kCIlp], le[kCIlp];
+#pragma unroll
+      for (int j = 0; j < kCIlp; ++j) {
+        lb[j] = le[j] = 0;
+        if (ok[j]) {
+          a.ci.list(a.lv_last, 0, w[j], lb[j], le[j]);
+          bytes += 28;
+        }
+        contrib += le[j] - lb[j];
+      }
+      for (uint32_t x = 0; x < a.lv_last.ninj; ++x) {
+        const uint32_t col = a.lv_last.inj[x];
+        if (col == k) continue;
+        uint64_t lo[kCIlp];
+        uint32_t n[kCIlp], v[kCIlp];
+        bool in_lds[kCIlp], f[kCIlp];
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j) {
+          v[j] = s_rows[r[j] * k + col];
+          lo[j] = lb[j];
+          n[j] = ok[j] && v[j] != w[j] ? static_cast<uint32_t>(le[j] - lb[j]) : 0u;
+          in_lds[j] = false;
+        }
+        find_mixed<kCIlp>(a.ci.cn_adj, s_rows, lo, n, v, in_lds, f, loads);
+#pragma unroll
+        for (int j = 0; j < kCIlp; ++j) contrib -= f[j] ? 1u : 0u;
+      }
+    }
+  }
+  const unsigned long long tot = block_sum(contrib, s_sum);
+  __syncthreads();  // s_sum is read again below
+  const unsigned long long b = block_sum(bytes + 4ull * loads, s_sum);
+  if (tid == 0) {  // summed by reduce_add2_kernel (one atomic per reducing workgroup, not per workgroup here)
+    a.tile_cnt[2 * blockIdx.x] = tot;
+    a.tile_cnt[2 * blockIdx.x + 1] = b;
+  }
+}
+
 // *out += sum of vals[0..n): a few workgroups (one atomic each, not one per tile).
 __global__ __launch_bounds__(256) void reduce_add_kernel(const uint64_t *vals, uint64_t n, unsigned long long *out) {
   __shared__ unsigned long long lds[256 / kWave];
@@ -245,6 +700,35 @@ __global__ __launch_bounds__(256) void reduce_add_kernel(const uint64_t *vals, u
     unsigned long long t = 0;
     for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
     if (t) atomicAdd(out, t);
+  }
+}
+
+// *out0 += sum of vals[2i], *out1 += sum of vals[2i + 1] over i < n.
+__global__ __launch_bounds__(256) void reduce_add2_kernel(const uint64_t *vals, uint64_t n, unsigned long long *out0,
+                                                          unsigned long long *out1) {
+  __shared__ unsigned long long lds[2][256 / kWave];
+  unsigned long long x = 0, y = 0;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+    x += vals[2 * i];
+    y += vals[2 * i + 1];
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    x += __shfl_xor(x, d, kWave);
+    y += __shfl_xor(y, d, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    lds[0][threadIdx.x / kWave] = x;
+    lds[1][threadIdx.x / kWave] = y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t0 = 0, t1 = 0;
+    for (int w = 0; w < 256 / kWave; ++w) {
+      t0 += lds[0][w];
+      t1 += lds[1][w];
+    }
+    if (t0) atomicAdd(out0, t0);
+    if (t1) atomicAdd(out1, t1);
   }
 }
 
@@ -405,7 +889,8 @@ enum : int {
   kCtlChunkB = 5,    // chunk choice for the current level (b, items, base)
   kCtlDegMark = 8,   // parent degrees summed over the rows of marked chunks
   kCtlDegCount = 9,  // and over the rows of counted chunks
-  kCtlSize = 10
+  kCtlCountBytes = 10,  // the count kernel's list lookups, searches, hub probes and LDS copies
+  kCtlSize = 11
 };
 
 // List mode.  Query vertex of each matching-order position, by value.
@@ -550,7 +1035,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   const uint64_t scan_n = std::max<uint64_t>({item_cap + 1, n1 + 1, words + 1ull, max_tiles});
   ws.ctl = static_cast<uint64_t *>(ws.get(8 * kCtlSize));
   ws.mask = static_cast<uint64_t *>(ws.get(8 * (item_cap / 64 + kTileWaves)));
-  ws.tiles = static_cast<uint64_t *>(ws.get(8 * max_tiles));
+  ws.tiles = static_cast<uint64_t *>(ws.get(16 * max_tiles));  // count kernel: (count, bytes) per workgroup
   ws.tile_row = static_cast<uint32_t *>(ws.get(4 * (max_tiles + 1)));
   ws.partials = static_cast<uint64_t *>(ws.get(8 * (scan_n / kScanTile + 2)));
   ws.rows[1] = static_cast<uint32_t *>(ws.get(4 * n1));
@@ -707,6 +1192,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     ea.tile_cnt = ws.tiles;
     ea.tile_off = ws.tiles;
     ea.count = reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCount);
+    ea.extra = reinterpret_cast<unsigned long long *>(ws.ctl + kCtlCountBytes);
     ea.item_base = next_base[k];
     ea.items = items;
     if (items == 0) {  // every row of the chunk has an empty driver list
@@ -734,6 +1220,8 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     // + candidate bitmap word) per item; the rows written are added below
     const uint64_t step_in = rows_in * (4ull * k + 9) + items * 4ull;
     if (k == stop && !want_list) {
+      // four tiles per workgroup; (count, bytes) per workgroup in ws.tiles
+      const uint32_t ngroups = ceil_div(ntiles, static_cast<uint32_t>(kCIlp));
       if (fused) {
         ea.lv_last = lj[L];
         ea.fused_rows = fused_rows;
@@ -741,15 +1229,14 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
         ea.fbase = ws.fbase;
         ea.fbeg = ws.fbeg;
         ea.flen = ws.flen;
-        kt.begin(CSM_K_COUNT);
-        expand_kernel<kFused><<<ntiles, kTile, lds_rows, s>>>(ea);
-        kt.end(CSM_K_COUNT, step_in);
-      } else {
-        kt.begin(CSM_K_COUNT);
-        expand_kernel<kCount><<<ntiles, kTile, lds_rows, s>>>(ea);
-        kt.end(CSM_K_COUNT, step_in);
       }
-      reduce_add_kernel<<<reduce_grid(ntiles), 256, 0, s>>>(ws.tiles, ntiles, ea.count);
+      kt.begin(CSM_K_COUNT);
+      if (fused && o.lds_lists) count_kernel<true, true><<<ngroups, kTile, lds_rows, s>>>(ea, ntiles);
+      else if (fused) count_kernel<true, false><<<ngroups, kTile, lds_rows, s>>>(ea, ntiles);
+      else if (o.lds_lists) count_kernel<false, true><<<ngroups, kTile, lds_rows, s>>>(ea, ntiles);
+      else count_kernel<false, false><<<ngroups, kTile, lds_rows, s>>>(ea, ntiles);
+      kt.end(CSM_K_COUNT, step_in);
+      reduce_add2_kernel<<<reduce_grid(ngroups), 256, 0, s>>>(ws.tiles, ngroups, ea.count, ea.extra);
       cursor[k] = b;
       st = next_chunk(k, b, true);
       if (st != CSM_OK) return st;
@@ -812,6 +1299,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
      // without the entries read by membership searches)
     CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
+    res.kernels[CSM_K_COUNT].bytes += pin[kCtlCountBytes];
     res.kernels[CSM_K_EXPAND].bytes_ref = ref_mark + 12ull * pin[kCtlDegMark];
     res.kernels[CSM_K_COUNT].bytes_ref = ref_count + 12ull * pin[kCtlDegCount];
     res.kernels[CSM_K_MATERIALIZE].bytes_ref = ref_mat;

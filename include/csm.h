@@ -195,7 +195,10 @@ typedef struct csm_match_opts {
   uint32_t symmetry;       /* 1 (count only): count one embedding per Aut(Q) class, as csm_query_symmetry's
                               conditions select; the result's automorphisms holds |Aut(Q)|, so the full
                               count is count * automorphisms (when not capped) */
-  uint32_t reserved;
+  uint32_t lds_lists;      /* 1 (count only): the count level's rows look up every anchor's candidate list
+                              and copy the short lists that many of their items search into LDS, where the
+                              items' membership searches run (default 0: hub bitmaps and adjacency
+                              searches, which measured faster on the synthetic workloads) */
 } csm_match_opts;
 
 /* Kernel classes for per-kernel accounting. */

@@ -1,7 +1,7 @@
 // Runs the library, built for the CPU emulation, on graph files:
 //   driver <data.graph> <cap> <query.graph>...
 // prints per query "plain <count> <capped>", the same from csm_match_noalloc
-// ("noalloc ..."), without a cap the count with symmetry breaking times
+// ("noalloc ...") and with the count level's lists in LDS ("plainl ..."), without a cap the count with symmetry breaking times
 // |Aut(Q)| ("plains ..."), then its list under a cap of 64
 // ("list <rows>" and one "row ..." line per embedding, sorted, query-id
 // order), "batch <count> <capped>", and with the candidate-aware order
@@ -50,6 +50,10 @@ int main(int argc, char **argv) {
     std::printf("plain %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
     check(csm_match_noalloc(ctx, g, q, &o, &r), "match noalloc", ctx);
     std::printf("noalloc %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
+    csm_match_opts lo2 = o;  // the count level's lists copied into LDS
+    lo2.lds_lists = 1;
+    check(csm_match(ctx, g, q, &lo2, &r), "match lds lists", ctx);
+    std::printf("plainl %llu %u\n", static_cast<unsigned long long>(r.count), r.capped);
     if (cap == 0) {  // symmetry breaking: one embedding per automorphism class
       csm_match_opts so = o;
       so.symmetry = 1;

@@ -52,6 +52,7 @@ struct Block {
   std::vector<uint64_t> wave_act;
   std::vector<char> lds;
   const char *kname = "";
+  uint64_t bar_or = 0;  // OR of the values the lanes brought to the last block barrier
 };
 
 thread_local Block *t_blk = nullptr;
@@ -178,8 +179,12 @@ void run_block(Block *b) {
     if (live == 0) return;
     if (at_bar == live) {
       if (bmixed) die(b, "lanes wait at different __syncthreads");
+      b->bar_or = 0;
       for (uint32_t i = 0; i < n; ++i)
-        if (b->lanes[i].st == kBlock) b->lanes[i].st = kRun;
+        if (b->lanes[i].st == kBlock) {
+          b->bar_or |= b->lanes[i].val;
+          b->lanes[i].st = kRun;
+        }
       continue;
     }
     die(b, "deadlock");
@@ -202,7 +207,18 @@ void emu_syncthreads(int site) {
   Lane &l = b->lanes[b->cur];
   l.st = kBlock;
   l.site = site;
+  l.val = 0;
   to_sched(b);
+}
+
+int emu_syncthreads_or(int site, int p) {
+  Block *b = t_blk;
+  Lane &l = b->lanes[b->cur];
+  l.st = kBlock;
+  l.site = site;
+  l.val = p != 0;
+  to_sched(b);
+  return b->bar_or != 0;
 }
 
 void emu_yield() {
@@ -415,6 +431,16 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
   const char *c = std::getenv("EMU_CUS");
   p->multiProcessorCount = c ? std::atoi(c) : 2;
   p->totalGlobalMem = 8ull << 30;
+  p->maxSharedMemoryPerMultiProcessor = 160 * 1024;
+  return hipSuccess;
+}
+// Static LDS of a kernel: EMU_STATIC_LDS bytes (default 36000, near the
+// persistent join kernel's), so that LDS budgets computed from it stay small
+// and their overflow paths run.
+hipError_t hipFuncGetAttributes(hipFuncAttributes *a, const void *) {
+  const char *c = std::getenv("EMU_STATIC_LDS");
+  a->sharedSizeBytes = c ? static_cast<size_t>(std::atol(c)) : 36000;
+  a->maxThreadsPerBlock = 1024;
   return hipSuccess;
 }
 hipError_t hipMemGetInfo(size_t *f, size_t *t) {

@@ -49,7 +49,13 @@ struct hipDeviceProp_t {
   char gcnArchName[256];
   int multiProcessorCount;
   size_t totalGlobalMem;
+  size_t maxSharedMemoryPerMultiProcessor;
 };
+struct hipFuncAttributes {
+  size_t sharedSizeBytes;
+  int maxThreadsPerBlock;
+};
+hipError_t hipFuncGetAttributes(hipFuncAttributes *a, const void *f);
 
 // ---------------------------------------------------------------- runtime API
 hipError_t hipMalloc(void **p, size_t n);
@@ -115,6 +121,8 @@ uint32_t emu_lane();
 uint64_t emu_memrealtime();
 
 #define __syncthreads() emu_syncthreads(__LINE__)
+int emu_syncthreads_or(int site, int p);
+#define __syncthreads_or(p) emu_syncthreads_or(__LINE__, (p))
 #define __threadfence() std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_fence(o, s) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_wave_barrier() (void)emu_wave_meet(__LINE__, 0, nullptr)

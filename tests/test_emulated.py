@@ -5,7 +5,8 @@ workgroup runs on an OS thread and its lanes on fibers, with block barriers
 and wave operations as meeting points (tests/emu/emu_rt.cpp).  Every global
 allocation is its own heap block, so an out-of-bounds access in a kernel is
 reported with its stack.  The plain path (also through csm_match_noalloc,
-its workspace in the static scratch) and the persistent batched kernel
+its workspace in the static scratch, and with the count level's lists
+copied into LDS) and the persistent batched kernel
 run on small random graphs and queries (with non-tree edges) and must give
 the oracle's counts, capped and uncapped, also with the candidate-aware
 order (reorder option), and the plain path's list under a cap of 64 (sorted
@@ -57,7 +58,8 @@ def _oracle_lists(dl, de, queries, cap=64):
 
 def _parse(stdout):
     """Per-kind results and the listed rows of each query, from the driver."""
-    got = {"plain": [], "noalloc": [], "plains": [], "batch": [], "plainr": [], "batchr": [], "served": []}
+    got = {"plain": [], "noalloc": [], "plainl": [], "plains": [], "batch": [], "plainr": [], "batchr": [],
+           "served": []}
     lists = []
     for ln in stdout.split("\n"):
         f = ln.split()
@@ -94,7 +96,7 @@ def test_device_code_on_cpu_matches_oracle(driver, tmp_path, seed):
         assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
         got, lists = _parse(out.stdout)
         want = _oracle(dl, de, queries, cap)
-        for kind in ("plain", "noalloc", "batch", "plainr", "batchr", "served") + (("plains",) if cap == 0 else ()):
+        for kind in ("plain", "noalloc", "plainl", "batch", "plainr", "batchr", "served") + (("plains",) if cap == 0 else ()):
             assert got[kind] == want, (cap, kind, got[kind], want)
         assert lists == _oracle_lists(dl, de, queries), cap
 
@@ -141,6 +143,37 @@ def test_heavy_candidate_lists_on_cpu(driver, tmp_path):
         assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
         got, lists = _parse(out.stdout)
         want = _oracle(dl, de, queries, cap)
-        for kind in ("plain", "noalloc", "batch", "plainr", "batchr", "served") + (("plains",) if cap == 0 else ()):
+        for kind in ("plain", "noalloc", "plainl", "batch", "plainr", "batchr", "served") + (("plains",) if cap == 0 else ()):
             assert got[kind] == want, (cap, kind, got[kind], want)
         assert lists == _oracle_lists(dl, de, queries), cap
+
+
+def test_dense_rows_lds_lists_on_cpu(driver, tmp_path):
+    """A dense graph without hubs (160 vertices of degree near 24, two labels)
+    and queries with non-tree edges: the count level's rows have many items
+    and short lists, so the lds_lists option copies lists into LDS; every path
+    gives the oracle's counts."""
+    r = random.Random(1)
+    n = 160
+    edges = set()
+    while len(edges) < n * 12:
+        a, b = r.sample(range(n), 2)
+        edges.add((min(a, b), max(a, b)))
+    de = sorted(edges)
+    dl = [r.randrange(2) for _ in range(n)]
+    queries = [rand_query(r, r.randint(3, 5), r.randint(1, 3), 2) for _ in range(4)]
+    dpath = tmp_path / "data.graph"
+    csm.HostGraph.from_edges(dl, de).write(str(dpath))
+    qpaths = []
+    for i, (ql, qe) in enumerate(queries):
+        p = tmp_path / f"q{i}.graph"
+        csm.HostGraph.from_edges(ql, qe).write(str(p))
+        qpaths.append(str(p))
+    env = dict(os.environ, EMU_THREADS="4", EMU_CUS="2", ASAN_OPTIONS="detect_leaks=0")
+    out = subprocess.run([driver, str(dpath), "0", *qpaths], env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-3000:]
+    assert "AddressSanitizer" not in out.stderr, out.stderr[-3000:]
+    got, _ = _parse(out.stdout)
+    want = _oracle(dl, de, queries, 0)
+    for kind in ("plain", "noalloc", "plainl", "plains", "batch", "plainr", "batchr", "served"):
+        assert got[kind] == want, (kind, got[kind], want)

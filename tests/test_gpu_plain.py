@@ -286,8 +286,51 @@ def test_hub_probes_and_fused_counts(ctx):
         assert (res.count, bool(res.capped)) == (oc, ocap), it
         res, _ = q.match(cap=50_000_000, chunk_items=4096)
         assert (res.count, bool(res.capped)) == (oc, ocap), it
+        res, _ = q.match(cap=50_000_000, lds_lists=True)
+        assert (res.count, bool(res.capped)) == (oc, ocap), it
         q.free()
     assert seen_fused and seen_back
+    g.free()
+
+
+def test_lds_lists_vs_oracle(ctx):
+    """The count level with its rows' short candidate lists copied into LDS
+    (lds_lists) gives the oracle's counts: a dense graph without hubs (degrees
+    near 48, two labels), so rows have many items and short non-hub lists,
+    queries with non-tree edges and with fused last positions, capped and
+    uncapped; and the default count on R-MAT 16's cyclic query."""
+    r = random.Random(31)
+    n = 1200
+    edges = set()
+    while len(edges) < n * 24:
+        a, b = r.sample(range(n), 2)
+        edges.add((min(a, b), max(a, b)))
+    de = sorted(edges)
+    dl = [r.randrange(2) for _ in range(n)]
+    d = O.Data.from_arrays(dl, de)
+    g = ctx.upload_edges(dl, de)
+    assert g.info()[2] < 128  # no hubs: every anchor is searched
+    for it in range(12):
+        ql, qe = rand_query(r, r.randint(3, 5), r.randint(0, 3), 2)
+        oq = O.Query.from_arrays(ql, qe)
+        cap = 20_000_000 if it % 2 else 0
+        oc, ocap, _, _, _ = O.match(d, oq, oq.plan(), cap=cap if cap else (1 << 64) - 1, threads=8)
+        q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+        res, _ = q.match(cap=cap, lds_lists=True)
+        print("lds lists", it, len(ql), len(qe), oc, bool(ocap), flush=True)
+        assert (res.count, bool(res.capped)) == (oc, bool(ocap)), it
+        res0, _ = q.match(cap=cap)
+        assert (res0.count, bool(res0.capped)) == (oc, bool(ocap)), it
+        q.free()
+    g.free()
+    hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
+    g = ctx.upload(hg)
+    q = g.prepare(csm.gen_query(hg, 8, csm.QUERY_EXTRA, 12, extra_edges=3).plan())
+    a, _ = q.match(cap=3_000_000_000)
+    b, _ = q.match(cap=3_000_000_000, lds_lists=True)
+    print("rmat16 cycles+3", a.count, b.count, a.items, b.items, flush=True)
+    assert (a.count, a.capped, a.items) == (b.count, b.capped, b.items)
+    q.free()
     g.free()
 
 
