@@ -735,30 +735,15 @@ __global__ __launch_bounds__(256) void reduce_add2_kernel(const uint64_t *vals, 
 uint32_t reduce_grid(uint64_t n) { return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(256, n / 2048))); }
 
 // tile_row[t] = row (relative to a) holding item t * kTile of the chunk
-// [a, b); tile_row[ntiles] = b - a.  For the byte accounting, *deg_acc gains
-// the degree of each row's vertex at column `parent` (one atomic per block).
+// [a, b); tile_row[ntiles] = b - a.
 __global__ __launch_bounds__(256) void tile_rows_kernel(const uint64_t *dscan, uint64_t a, uint64_t b, uint64_t ntiles,
-                                                        uint32_t *tile_row, const uint32_t *rows, uint32_t k,
-                                                        uint32_t parent, const uint32_t *off,
-                                                        unsigned long long *deg_acc) {
-  __shared__ unsigned long long lds[256 / kWave];
+                                                        uint32_t *tile_row) {
   const uint64_t base = dscan[a];
-  unsigned long long deg = 0;
   for (uint64_t r = a + blockIdx.x * 256ull + threadIdx.x; r < b; r += 256ull * gridDim.x) {
     const uint64_t lo = dscan[r] - base, hi = dscan[r + 1] - base;  // items [lo, hi)
     for (uint64_t t = (lo + kTile - 1) / kTile; t * kTile < hi; ++t) tile_row[t] = static_cast<uint32_t>(r - a);
-    const uint32_t x = rows[r * k + parent];
-    deg += off[x + 1] - off[x];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) tile_row[ntiles] = static_cast<uint32_t>(b - a);
-  for (int d = kWave / 2; d > 0; d >>= 1) deg += __shfl_xor(deg, d, kWave);
-  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = deg;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
-    if (t) atomicAdd(deg_acc, t);
-  }
 }
 
 // Per-row pass of a level.
@@ -770,7 +755,8 @@ __global__ __launch_bounds__(256) void tile_rows_kernel(const uint64_t *dscan, u
 // and its length minus the row's own members.  With row_count (one anchor,
 // and no per-item correction), the row's count is its driver list minus the
 // row's own ids, times fbase when fused: it goes to partials[block] and the
-// row gets no items (dscan[i] = 0).
+// row gets no items (dscan[i] = 0).  With deg_sums, for the reference-byte
+// accounting, deg_sums[block] = the degrees of the rows' parent vertices.
 struct RowPass {
   LevelJoin lv_last;
   uint32_t fused_rows, row_count;
@@ -779,12 +765,14 @@ struct RowPass {
   uint64_t *fbeg;
   uint32_t *flen;
   uint64_t *partials;  // row_count: one per workgroup
+  const uint32_t *off;  // graph offsets, for deg_sums
+  uint64_t *deg_sums;   // one per workgroup, or null
 };
 
 __global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uint32_t *rows, uint32_t k, LevelJoin lv,
                                                          uint64_t n, uint64_t *dscan, uint8_t *drv, RowPass rp) {
   __shared__ unsigned long long lds[256 / kWave];
-  unsigned long long acc = 0;
+  unsigned long long acc = 0, deg = 0;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i <= n; i += 256ull * gridDim.x) {
     if (i == n) {
       dscan[i] = 0;
@@ -830,6 +818,10 @@ __global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uin
     }
     dscan[i] = best;
     drv[i] = static_cast<uint8_t>(bi);
+    if (rp.deg_sums) {
+      const uint32_t x = row[lv.anchor[0]];  // the parent's vertex
+      deg += rp.off[x + 1] - rp.off[x];
+    }
   }
   if (rp.row_count) {
     for (int d = kWave / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, kWave);
@@ -839,6 +831,17 @@ __global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uin
       unsigned long long t = 0;
       for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
       rp.partials[blockIdx.x] = t;
+    }
+  }
+  if (rp.deg_sums) {
+    __syncthreads();  // lds may still be read above
+    for (int d = kWave / 2; d > 0; d >>= 1) deg += __shfl_xor(deg, d, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = deg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int w = 0; w < 256 / kWave; ++w) t += lds[w];
+      rp.deg_sums[blockIdx.x] = t;
     }
   }
 }
@@ -946,6 +949,7 @@ struct Workspace {
   uint32_t *tile_row = nullptr;
   uint32_t *fbase = nullptr, *flen = nullptr;  // count level, per row
   uint64_t *fbeg = nullptr, *row_sums = nullptr;
+  uint64_t *deg_sums = nullptr;  // row pass: parent degrees per workgroup
   void *get(uint64_t bytes) { return mem->get(bytes); }
 };
 
@@ -1041,7 +1045,9 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
   ws.rows[1] = static_cast<uint32_t *>(ws.get(4 * n1));
   ws.dscan[1] = static_cast<uint64_t *>(ws.get(8 * std::max<uint64_t>(n1 + 1, words + 1ull)));
   ws.drv[1] = static_cast<uint8_t *>(ws.get(n1 + 1));
-  bool okalloc = ws.ctl && ws.mask && ws.tiles && ws.tile_row && ws.partials && ws.rows[1] && ws.dscan[1] && ws.drv[1];
+  ws.deg_sums = static_cast<uint64_t *>(ws.get(8 * 65536));  // one per row-pass workgroup (grid_for's bound)
+  bool okalloc = ws.ctl && ws.mask && ws.tiles && ws.tile_row && ws.partials && ws.rows[1] && ws.dscan[1] && ws.drv[1] &&
+                 ws.deg_sums;
   for (uint32_t k = 2; k <= stop && okalloc; ++k) {
     ws.rows[k] = static_cast<uint32_t *>(ws.get(4ull * k * item_cap));
     ws.dscan[k] = static_cast<uint64_t *>(ws.get(8ull * (item_cap + 1)));
@@ -1134,11 +1140,18 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
       cursor[k] = nrows[k];
       return CSM_OK;
     }
+    // the reference-byte accounting's parent degrees, over every row of the
+    // level: the rows of counted levels and of marked ones
+    rp.off = g->off;
+    rp.deg_sums = ws.deg_sums;
+    const bool counted = k == stop && !want_list;
     kt.begin(CSM_K_AUX);
     anchor_len_kernel<<<grid, 256, 0, s>>>(ci, ws.rows[k], k, lj[k], nrows[k], ws.dscan[k], ws.drv[k], rp);
+    reduce_add_kernel<<<reduce_grid(grid), 256, 0, s>>>(
+        ws.deg_sums, grid, reinterpret_cast<unsigned long long *>(ws.ctl + (counted ? kCtlDegCount : kCtlDegMark)));
     scan_u64_exclusive(ws.dscan[k], nrows[k] + 1, ws.partials, nullptr, s);
     choose_chunk_kernel<<<1, 64, 0, s>>>(ws.dscan[k], nrows[k], 0, item_cap, ws.ctl + kCtlChunkA);
-    kt.end(CSM_K_AUX, nrows[k] * (per_row + 9 + (count_level ? (last_inj_k ? 16 : 4) : 0)));
+    kt.end(CSM_K_AUX, nrows[k] * (per_row + 9 + 8 + (count_level ? (last_inj_k ? 16 : 4) : 0)));
     CSM_HIP_TRY(ctx, hipMemcpyAsync(pin, ws.ctl, 8 * kCtlSize, hipMemcpyDeviceToHost, s));
     CSM_HIP_TRY(ctx, hipStreamSynchronize(s));
     cursor[k] = 0;
@@ -1208,9 +1221,7 @@ csm_status run_join(csm_ctx *ctx, const csm_graph *g, csm_query *q, const csm_ma
     const size_t lds_rows = 4ull * kTileRows * k;
     const bool counted = k == stop && !want_list;
     kt.begin(CSM_K_AUX);
-    tile_rows_kernel<<<grid_for(rows_in), 256, 0, s>>>(
-        ws.dscan[k], a, b, ntiles, ws.tile_row, ws.rows[k], k, lv[k].parent, g->off,
-        reinterpret_cast<unsigned long long *>(ws.ctl + (counted ? kCtlDegCount : kCtlDegMark)));
+    tile_rows_kernel<<<grid_for(rows_in), 256, 0, s>>>(ws.dscan[k], a, b, ntiles, ws.tile_row);
     kt.end(CSM_K_AUX, rows_in * 16ull + 4ull * ntiles);
     // the reference's extension pass over these rows, without the scan of the
     // parents' adjacency (added from the device sums at the end)
