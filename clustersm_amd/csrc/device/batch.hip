@@ -359,13 +359,14 @@ constexpr uint32_t kCache = 16;  // free blocks a workgroup keeps for itself
 constexpr uint32_t kDefaultBlocksPerWg = 16;  // pool blocks per workgroup the default block size keeps
 
 static_assert(kRowBatch <= 256 && kRowBatch <= kThreads, "row index in a batch is a byte");
+// dynamic LDS words at least: a row batch of one column, then a root task's ids
+constexpr uint32_t kRootLdsWords = kRowBatch + kMaxRootWords * 64;
 
 struct Smem {
   Task task;
   BQuery q;
   LevelJoin lv;
   LevelJoin lv_last;
-  uint32_t roots[kMaxRootWords * 64];
   uint64_t pre[kRowBatch + 1];
   // candidate list of each kept anchor, anchor-major so that lanes reading
   // one anchor of different rows hit consecutive banks: begin
@@ -699,6 +700,9 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
   // rows of the task; a stolen task's rows were written on another workgroup
   // (possibly another XCD): staged with agent-scope loads
   const bool stolen = sm.task_stolen != 0;
+  // a root task's ids, after the first row batch's single column in the
+  // dynamic LDS (which holds at least kRootLdsWords words)
+  uint32_t *s_roots = s_rows + kRowBatch;
   uint32_t nrows;
   uint32_t *grows = nullptr;
   if (k == 1) {
@@ -716,7 +720,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     while (w) {
       const int i = __builtin_ctzll(w);
       w &= w - 1;
-      sm.roots[p++] = (t.r0 + tid) * 64u + static_cast<uint32_t>(i);
+      s_roots[p++] = (t.r0 + tid) * 64u + static_cast<uint32_t>(i);
     }
     nrows = static_cast<uint32_t>(tot);
     __syncthreads();
@@ -755,7 +759,7 @@ __device__ void process_task(const BArgs &a, Smem &sm, uint32_t *s_rows, uint32_
     const uint32_t nb = min(kRowBatch, nrows - row0);
     const uint32_t nwords = nb * k;
     if (k == 1) {
-      for (uint32_t i = tid; i < nwords; i += kThreads) s_rows[i] = sm.roots[row0 + i];
+      for (uint32_t i = tid; i < nwords; i += kThreads) s_rows[i] = s_roots[row0 + i];
     } else {
       // kRowLoads loads per thread issued before any is waited for
       const uint32_t *src = grows + static_cast<uint64_t>(row0) * k;
@@ -1793,9 +1797,15 @@ csm_status BatchRun::launch(hipStream_t s) {
   a.deadline_ticks = static_cast<uint64_t>(limit_s * 1e8);  // 100 MHz real-time clock
   grid_ = grid;
   num_blocks_ = num_blocks;
-  const size_t lds_rows = 4ull * kRowBatch * std::max<uint32_t>(1, max_nq_ - 1);
+  const size_t lds_rows = 4ull * std::max<uint32_t>(kRowBatch * std::max<uint32_t>(1, max_nq_ - 1), kRootLdsWords);
   auto launch_join = [&]() {
+    // register budgets so that wgs_per_cu workgroups of 4 waves fit a CU's
+    // 4 SIMDs (512 VGPRs per lane each): 128 (4 per CU), 168 (3), 256 (2).
+    // More than 4 per CU needs at most 96 VGPRs: the spills made 5 per CU
+    // 1.2x slower on the headline and R-MAT 20 batches, so 4 is the most.
+    // Measured: 3 per CU (168 VGPRs, 4 spills) 1.09x slower than 4.
     if (tp.wgs_per_cu >= 4) batch_join_kernel<4><<<grid, kThreads, lds_rows, s>>>(a);
+    else if (tp.wgs_per_cu == 3) batch_join_kernel<3><<<grid, kThreads, lds_rows, s>>>(a);
     else batch_join_kernel<2><<<grid, kThreads, lds_rows, s>>>(a);
   };
   join_kt_.reset(new KernelTimer(ctx, kstat_.data()));
@@ -2027,7 +2037,7 @@ csm_status ResidentJoin::launch() {
   BArgs &a = *static_cast<BArgs *>(args_);
   a.epoch = g_launches.fetch_add(1) + 1;
   const uint32_t grid = ctx_->num_cus * wgs_per_cu_;
-  const size_t lds_rows = 4ull * kRowBatch * (CSM_MAX_QV - 1);
+  const size_t lds_rows = 4ull * std::max<uint32_t>(kRowBatch * (CSM_MAX_QV - 1), kRootLdsWords);
   batch_init_kernel<<<grid_for(std::max<uint64_t>(a.num_blocks, grid)), 256, 0, s_>>>(a, grid);
   batch_join_kernel<4><<<grid, kThreads, lds_rows, s_>>>(a);
   CSM_HIP_TRY(ctx_, hipGetLastError());
