@@ -570,8 +570,9 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
             done += len(batch)
         span = time.perf_counter() - ts
         print(f"# served at {rate} queries/s: {len(sizes)} batches in {span:.2f}s", file=sys.stderr, flush=True)
-        return {"offered_qps": rate,
+        return {"offered_qps": rate, "arrival_span_s": round(arr[-1], 3),
                 "batched": {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "queries_per_s": round(len(keep) / span, 2),
+                            "drain_ms": round(1e3 * (span - arr[-1]), 1),
                             "batches": len(sizes), "mean_batch": round(sum(sizes) / max(1, len(sizes)), 1),
                             "batches_of_one_on_plain_path": singles},
                 "plain_fifo": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
@@ -607,8 +608,12 @@ def served_section(ctx, nqueries, qps, cap, limit_ms, max_batch, budget_gb, seed
             if (x.status, x.count, x.capped) != (0, p.count, p.capped):
                 mism.append([i, x.count, int(x.capped), x.status, "cpp_server", p.count, int(p.capped), rate])
         print(f"# C++ server at {rate} queries/s: {span:.2f}s", file=sys.stderr, flush=True)
+        # queries/s over first arrival to last result is bounded by the
+        # stream's own span (len / arr[-1]); drain_ms: last result after the
+        # last arrival
         return {"offered_qps": rate, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                "queries_per_s": round(len(keep) / span, 2)}
+                "queries_per_s": round(len(keep) / span, 2), "arrival_span_s": round(arr[-1], 3),
+                "arrival_bound_qps": round(len(keep) / arr[-1], 2), "drain_ms": round(1e3 * (span - arr[-1]), 1)}
 
     rates = [qps] + list(extra_qps or [])
     cpp = [run_cpp(x, random.Random(seed + 201 + k)) for k, x in enumerate(rates)]
