@@ -122,6 +122,8 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   __shared__ uint64_t sd[kTileRows + 1];
   __shared__ uint64_t s_base[kTileRows];
   __shared__ uint32_t s_wcnt[kTileWaves];
+  __shared__ uint16_t s_ori[MODE == kMaterialize ? kTile : 1];  // MATERIALIZE: survivors' staged rows
+  __shared__ uint32_t s_ow[MODE == kMaterialize ? kTile : 1];   // and new ids, in output order
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kTile;
   const uint64_t rlo = a.row_a + a.tile_row[blockIdx.x];
@@ -147,6 +149,7 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   const bool valid = item < a.items;
   bool ok = false;
   uint32_t w = 0;
+  uint32_t rs = 0;  // the item's staged row
   const uint32_t *row = nullptr;
   if (valid) {
     const uint64_t gi = a.item_base + item;
@@ -157,6 +160,7 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
         const uint32_t mid = (lo + hi) >> 1;
         if (sd[mid] <= gi) lo = mid; else hi = mid;
       }
+      rs = lo;
       row = s_rows + lo * k;
       d = a.drv[rlo + lo];
       w = a.ci.cn_adj[s_base[lo] + (gi - sd[lo])];
@@ -183,12 +187,34 @@ __global__ __launch_bounds__(kTile) void expand_kernel(ExpandArgs a) {
   if (MODE == kMaterialize) {
     if (lane == 0) s_wcnt[wave] = static_cast<uint32_t>(__popcll(m));
     __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int x = 0; x < kTileWaves; ++x) {
+      before += x < wave ? s_wcnt[x] : 0u;
+      tot += s_wcnt[x];
+    }
+    if (wide) {  // rows in global memory: each survivor writes its own row
+      if (ok) {
+        uint32_t *dst = a.out + (a.tile_off[blockIdx.x] + before + lanes_below(m)) * (k + 1);
+        for (uint32_t j = 0; j < k; ++j) dst[j] = row[j];
+        dst[k] = w;
+      }
+      return;
+    }
+    // survivors (staged row, new id) to LDS in output order, then the tile's
+    // rows leave as one contiguous, coalesced run of words
     if (ok) {
-      uint64_t pos = a.tile_off[blockIdx.x] + lanes_below(m);
-      for (int x = 0; x < wave; ++x) pos += s_wcnt[x];
-      uint32_t *dst = a.out + pos * (k + 1);
-      for (uint32_t j = 0; j < k; ++j) dst[j] = row[j];
-      dst[k] = w;
+      const uint32_t p = before + lanes_below(m);
+      s_ori[p] = static_cast<uint16_t>(rs);
+      s_ow[p] = w;
+    }
+    __syncthreads();
+    const uint32_t kout = k + 1;
+    const uint32_t kout_magic = static_cast<uint32_t>((0x100000000ull + kout - 1) / kout);  // i / kout for i < 2^17
+    uint32_t *dst = a.out + a.tile_off[blockIdx.x] * kout;
+    const uint32_t nw = tot * kout;
+    for (uint32_t i = tid; i < nw; i += kTile) {
+      const uint32_t r = __umulhi(i, kout_magic), c = i - r * kout;
+      dst[i] = c < k ? s_rows[s_ori[r] * k + c] : s_ow[r];
     }
     return;
   }
