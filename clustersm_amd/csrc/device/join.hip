@@ -81,16 +81,6 @@ struct ExpandArgs {
 
 enum { kMark = 0, kMaterialize = 2 };
 
-// v in the sorted list cn[beg .. beg + len)
-__device__ inline bool in_list(const uint32_t *cn, uint64_t beg, uint64_t len, uint32_t v) {
-  while (len > 1) {
-    const uint64_t half = len >> 1;
-    if (cn[beg + half] <= v) beg += half;
-    len -= half;
-  }
-  return len == 1 && cn[beg] == v;
-}
-
 template <typename T>
 __device__ inline T block_sum(T v, T *lds) {
   for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
@@ -805,9 +795,38 @@ __global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uin
       continue;
     }
     const uint32_t *row = rows + i * k;
+    // the first kRowLists anchors' lists and the fused count's list are
+    // looked up together: every rank word and prefix in one round, then every
+    // start and length in a second
+    constexpr uint32_t kRowLists = 4;
+    const uint32_t na = min(lv.nanchor, kRowLists);
+    uint64_t ix[kRowLists + 1];
+#pragma unroll
+    for (uint32_t t = 0; t <= kRowLists; ++t) {
+      ix[t] = 0;
+      if (t < na) ix[t] = lv.off_base[t] + ci.rank(lv.src_u[t], row[lv.anchor[t]]);
+      else if (t == kRowLists && rp.fused_rows)
+        ix[t] = rp.lv_last.off_base[0] + ci.rank(rp.lv_last.src_u[0], row[rp.lv_last.parent]);
+    }
+    uint64_t lb[kRowLists + 1];
+    uint32_t ln[kRowLists + 1];
+#pragma unroll
+    for (uint32_t t = 0; t <= kRowLists; ++t) {
+      const bool use = t < na || (t == kRowLists && rp.fused_rows);
+      lb[t] = use ? ci.cn_off[ix[t]] : 0;
+      ln[t] = use ? ci.cn_len[ix[t]] : 0;
+    }
     uint64_t best = ~0ull, bbeg = 0;
     uint32_t bi = 0;
-    for (uint32_t t = 0; t < lv.nanchor; ++t) {
+#pragma unroll
+    for (uint32_t t = 0; t < kRowLists; ++t) {
+      if (t < na && ln[t] < best) {
+        best = ln[t];
+        bbeg = lb[t];
+        bi = t;
+      }
+    }
+    for (uint32_t t = kRowLists; t < lv.nanchor; ++t) {
       uint64_t beg, end;
       ci.list(lv, t, row[lv.anchor[t]], beg, end);
       if (end - beg < best) {
@@ -816,29 +835,56 @@ __global__ __launch_bounds__(256) void anchor_len_kernel(CandIndex ci, const uin
         bi = t;
       }
     }
+    unsigned long long loads = 0;  // not counted here: the host charges the row pass per row
     uint32_t fb = 1;
     if (rp.fused_rows) {
+      // the list minus the row's own members of it, searched in lockstep
       const uint32_t xv = row[rp.lv_last.parent];
-      uint64_t lb, le;
-      ci.list(rp.lv_last, 0, xv, lb, le);
-      uint64_t c = le - lb;
-      for (uint32_t t = 0; t < rp.lv_last.ninj; ++t) {
-        const uint32_t col = rp.lv_last.inj[t];
-        if (col == k) continue;
-        const uint32_t v = row[col];
-        if (v != xv && in_list(ci.cn_adj, lb, le - lb, v)) --c;
+      const uint64_t flb = lb[kRowLists];
+      const uint32_t fln = ln[kRowLists];
+      uint64_t c = fln;
+      for (uint32_t t0 = 0; t0 < rp.lv_last.ninj; t0 += 4) {
+        uint64_t lo[4];
+        uint32_t n4[4], v[4];
+        bool f[4], in_lds[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t t = t0 + j;
+          const uint32_t col = t < rp.lv_last.ninj ? rp.lv_last.inj[t] : k;
+          v[j] = col < k ? row[col] : 0u;
+          lo[j] = flb;
+          n4[j] = col < k && v[j] != xv ? fln : 0u;
+          in_lds[j] = false;
+        }
+        find_mixed<4>(ci.cn_adj, ci.cn_adj, lo, n4, v, in_lds, f, loads);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) c -= f[j] ? 1 : 0;
       }
       fb = static_cast<uint32_t>(c);
       rp.fbase[i] = fb;
       if (rp.store_list) {
-        rp.fbeg[i] = lb;
-        rp.flen[i] = static_cast<uint32_t>(le - lb);
+        rp.fbeg[i] = flb;
+        rp.flen[i] = fln;
       }
     }
-    if (rp.row_count) {
+    if (rp.row_count) {  // the driver list minus the row's own ids, searched in lockstep
       uint64_t surv = best;
-      for (uint32_t t = 0; t < lv.ninj; ++t)
-        if (in_list(ci.cn_adj, bbeg, best, row[lv.inj[t]])) --surv;
+      for (uint32_t t0 = 0; t0 < lv.ninj; t0 += 4) {
+        uint64_t lo[4];
+        uint32_t n4[4], v[4];
+        bool f[4], in_lds[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t t = t0 + j;
+          v[j] = t < lv.ninj ? row[lv.inj[t]] : 0u;
+          lo[j] = bbeg;
+          n4[j] = t < lv.ninj ? static_cast<uint32_t>(best) : 0u;
+          in_lds[j] = false;
+        }
+        find_mixed<4>(ci.cn_adj, ci.cn_adj, lo, n4, v, in_lds, f, loads);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) surv -= f[j] ? 1 : 0;
+      }
       acc += surv * fb;
       best = 0;
     }
