@@ -125,14 +125,22 @@ __global__ __launch_bounds__(kFilterThreads) void nlc_filter_kernel(const uint32
 }
 
 // The same predicate read from the graph's neighbour-label table: one wave
-// per 64 vertices, each lane reads its vertex's label and degree and, when
-// they can match, the bytes of its table row that the signatures name.
-// A coalesced stream instead of an adjacency scan with label gathers.
+// per 64 vertices, each lane reads its vertex's label and degree; a wave with
+// a vertex that can match copies its 64 table rows (4 KB, contiguous) into
+// LDS with four coalesced 16-byte loads per lane, and every query vertex's
+// signature is then checked against LDS.  A coalesced stream instead of an
+// adjacency scan with label gathers, and one round trip per 64 vertices
+// instead of one per query vertex.
 __global__ __launch_bounds__(kFilterThreads) void nlc_table_filter_kernel(const uint32_t *__restrict__ off,
                                                                           const uint32_t *__restrict__ label,
                                                                           const uint8_t *__restrict__ nlc, uint32_t n,
                                                                           const FilterQ *__restrict__ qs) {
+  static_assert(kNlcStride == 64, "a wave's 64 table rows are 256 16-byte pieces");
+  // rows at a stride of 17 words, so that the lanes' reads of one byte
+  // position of their own rows fall in different banks
+  constexpr uint32_t kRowWords = kNlcStride / 4 + 1;
   __shared__ FilterSmem sm;
+  __shared__ uint32_t s_row[kFilterWaves][kWave * kRowWords];
   const FilterQ fq = qs[blockIdx.y];
   if (!fq.use_table) return;
   const int tid = threadIdx.x;
@@ -150,34 +158,59 @@ __global__ __launch_bounds__(kFilterThreads) void nlc_table_filter_kernel(const 
   const int wave = tid / kWave, lane = tid % kWave;
   const uint32_t words = sm.d.words, nq = sm.d.nq;
   uint64_t *bits = fq.bits;
-  uint64_t rows = 0;
-  for (uint32_t b = blockIdx.x * kFilterWaves + wave; b < words; b += gridDim.x * kFilterWaves) {
+  uint64_t rows = 0;  // table rows copied (lane 0)
+  const uint8_t *my_row = reinterpret_cast<const uint8_t *>(s_row[wave] + lane * kRowWords);
+  // the same number of iterations in every wave of the workgroup (barriers)
+  for (uint32_t base = blockIdx.x * kFilterWaves; base < words; base += gridDim.x * kFilterWaves) {
+    const uint32_t b = base + static_cast<uint32_t>(wave);
+    const bool wv = b < words;
     const uint32_t v = b * 64u + static_cast<uint32_t>(lane);
     uint32_t s = kNoSlot, dv = 0;
-    if (v < n) {
+    if (wv && v < n) {
       s = lookup_slot(sm, label[v]);
       dv = off[v + 1] - off[v];
     }
     const bool active = s != kNoSlot && dv >= sm.d.slot_min_deg[s];
-    rows += active ? 1 : 0;
-    const uint8_t *row = nlc + static_cast<uint64_t>(v) * kNlcStride;
+    const uint64_t am = ballot(active);
+    if (am) {  // uniform in the wave
+      const uint32_t nrow = min(64u, n - b * 64u);  // rows of this word that exist
+      const uint4 *src = reinterpret_cast<const uint4 *>(nlc + static_cast<uint64_t>(b) * 64u * kNlcStride);
+      uint4 q[4];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t c = j * kWave + static_cast<uint32_t>(lane);  // piece c holds bytes of row c / 4
+        q[j] = c / 4 < nrow ? src[c] : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t c = j * kWave + static_cast<uint32_t>(lane);
+        uint32_t *d = s_row[wave] + (c / 4) * kRowWords + (c % 4) * 4;
+        d[0] = q[j].x;
+        d[1] = q[j].y;
+        d[2] = q[j].z;
+        d[3] = q[j].w;
+      }
+      rows += lane == 0 ? nrow : 0;
+    }
+    __syncthreads();
     for (uint32_t u = 0; u < nq; ++u) {
       bool ok = active && sm.d.slot[u] == s && sm.d.deg[u] <= dv;
       if (ok)
         for (uint32_t t = sm.d.sig_off[u]; t < sm.d.sig_off[u + 1]; ++t)
-          if (row[sm.d.slot_label[sm.d.sig_slot[t]]] < sm.d.sig_cnt[t]) {
+          if (my_row[sm.d.slot_label[sm.d.sig_slot[t]]] < sm.d.sig_cnt[t]) {
             ok = false;
             break;
           }
       const uint64_t m = ballot(ok);
-      if (lane == 0) {
+      if (lane == 0 && wv) {
         bits[static_cast<uint64_t>(u) * words + b] = m;
         if (m) atomicAdd(&sm.cnt[u], static_cast<unsigned long long>(__popcll(m)));
       }
     }
+    __syncthreads();  // the rows are overwritten by the next iteration
   }
-  // one table row (kNlcStride bytes) per vertex that could match, counted as
-  // 8-byte units in the scanned statistic
+  // table rows copied (kNlcStride bytes each), counted as 8-byte units in the
+  // scanned statistic
   for (int d = kWave / 2; d > 0; d >>= 1) rows += __shfl_xor(rows, d, kWave);
   __shared__ unsigned long long s_rows;
   if (tid == 0) s_rows = 0;
