@@ -431,16 +431,6 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
   const char *c = std::getenv("EMU_CUS");
   p->multiProcessorCount = c ? std::atoi(c) : 2;
   p->totalGlobalMem = 8ull << 30;
-  p->maxSharedMemoryPerMultiProcessor = 160 * 1024;
-  return hipSuccess;
-}
-// Static LDS of a kernel: EMU_STATIC_LDS bytes (default 36000, near the
-// persistent join kernel's), so that LDS budgets computed from it stay small
-// and their overflow paths run.
-hipError_t hipFuncGetAttributes(hipFuncAttributes *a, const void *) {
-  const char *c = std::getenv("EMU_STATIC_LDS");
-  a->sharedSizeBytes = c ? static_cast<size_t>(std::atol(c)) : 36000;
-  a->maxThreadsPerBlock = 1024;
   return hipSuccess;
 }
 hipError_t hipMemGetInfo(size_t *f, size_t *t) {

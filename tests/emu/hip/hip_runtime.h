@@ -49,13 +49,7 @@ struct hipDeviceProp_t {
   char gcnArchName[256];
   int multiProcessorCount;
   size_t totalGlobalMem;
-  size_t maxSharedMemoryPerMultiProcessor;
 };
-struct hipFuncAttributes {
-  size_t sharedSizeBytes;
-  int maxThreadsPerBlock;
-};
-hipError_t hipFuncGetAttributes(hipFuncAttributes *a, const void *f);
 
 // ---------------------------------------------------------------- runtime API
 hipError_t hipMalloc(void **p, size_t n);
