@@ -17,7 +17,7 @@ algorithmic bytes = the reference's extension pass over the same rows
 of the row's parent vertex, rows written), with the kernel's own logical
 bytes beside it.
 
-c2_full: the plain path on R-MAT scale 16 (65,536 vertices, 2^20 edge
+rmat16_full: the plain path on R-MAT scale 16 (65,536 vertices, 2^20 edge
 samples, a=0.57 b=0.19 c=0.19), 16 labels Zipf(1.0), two 8-vertex
 random-walk queries (tree-like; with 3 extra cycle edges), full enumeration
 (no cap) under a time limit per query: an unfinished query reports the
@@ -59,18 +59,18 @@ METRIC = "embeddings/s + batched queries/s vs per-query launch; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 
 
-C4_QUERIES = 128
-C4_CAP = 1_000_000_000
+HEADLINE_QUERIES = 128
+HEADLINE_CAP = 1_000_000_000
 
 
-def c4_workload():
+def chung_lu_16m_graph():
     """Chung-Lu 16M vertices / 256M edge samples, 64 Zipf(0.8) labels, and its
     128 random-walk queries (12..24 vertices, 2..6 extra edges)."""
     hg = csm.gen_chung_lu(16_000_000, 256_000_000, 2.1, 50_000, 64, 0.8, 41)
     return hg
 
 
-def c4_queries(host, n):
+def headline_queries(host, n):
     r = random.Random(4)
     out = []
     for i in range(n):
@@ -184,23 +184,23 @@ def cpu_baseline(hg, host_queries, cap, seconds):
     }
 
 
-def c4_section(ctx, args):
+def headline_section(ctx, args):
     """The headline: timed steps of batched groups, then one pass of the plain
     path and one whole-set batch call over the same 128 queries."""
     t0 = time.time()
-    hg = c4_workload()
+    hg = chung_lu_16m_graph()
     g = ctx.upload(hg)
     host = g.to_host()
-    hqs = c4_queries(host, C4_QUERIES)
+    hqs = headline_queries(host, HEADLINE_QUERIES)
     del host
     qs = [g.prepare(h.plan()) for h in hqs]
     nv, ne, maxdeg = g.info()
-    print(f"# C4 setup {time.time() - t0:.1f}s: |V|={nv} |E|={ne} max_degree={maxdeg}", file=sys.stderr, flush=True)
+    print(f"# headline setup {time.time() - t0:.1f}s: |V|={nv} |E|={ne} max_degree={maxdeg}", file=sys.stderr, flush=True)
     group = args.group
     groups = [qs[i:i + group] for i in range(0, len(qs), group)]
 
     def step(i):
-        return csm.batch_match(g, groups[i % len(groups)], cap=C4_CAP)
+        return csm.batch_match(g, groups[i % len(groups)], cap=HEADLINE_CAP)
 
     for i in range(args.warmup):
         step(i)
@@ -212,19 +212,19 @@ def c4_section(ctx, args):
     embeddings = sum(x.count for st in steps for x in st)
     anomalies = sum(x.anomalies for st in steps for x in st)
     recounted = sum(1 for st in steps for x in st if x.path in (csm.PATH_BATCH_BUDGET, csm.PATH_BATCH_ABORT))
-    print(f"# C4 timed {args.steps} steps in {elapsed:.2f}s: {embeddings} embeddings", file=sys.stderr, flush=True)
+    print(f"# headline timed {args.steps} steps in {elapsed:.2f}s: {embeddings} embeddings", file=sys.stderr, flush=True)
     tot = kernel_totals([[st[0]] for st in steps])  # a batch call's kernel totals are on its first result
     # one pass of the plain path, and one batch call over all 128
     plain, t_plain = [], 0.0
     for q in qs:
         t2 = time.perf_counter()
-        plain.append(q.match(cap=C4_CAP)[0])
+        plain.append(q.match(cap=HEADLINE_CAP)[0])
         t_plain += time.perf_counter() - t2
-    print(f"# C4 plain path: {len(qs)} queries in {t_plain:.2f}s", file=sys.stderr, flush=True)
+    print(f"# headline plain path: {len(qs)} queries in {t_plain:.2f}s", file=sys.stderr, flush=True)
     t3 = time.perf_counter()
-    whole = csm.batch_match(g, qs, cap=C4_CAP)
+    whole = csm.batch_match(g, qs, cap=HEADLINE_CAP)
     t_whole = time.perf_counter() - t3
-    print(f"# C4 one batch call: {len(qs)} queries in {t_whole:.2f}s", file=sys.stderr, flush=True)
+    print(f"# headline one batch call: {len(qs)} queries in {t_whole:.2f}s", file=sys.stderr, flush=True)
     mism = [i for i, (x, p) in enumerate(zip(whole, plain)) if (x.count, x.capped, x.status) != (p.count, p.capped, 0)]
     by_step = {}
     for i, st in enumerate(steps):
@@ -261,7 +261,7 @@ def c4_section(ctx, args):
     return out
 
 
-def c2_section(ctx, limits_ms):
+def rmat16_section(ctx, limits_ms):
     """The plain path on R-MAT 16 with the two 8-vertex queries, no cap."""
     hg = csm.gen_rmat(16, 1 << 20, 16, 1.0, 1)
     g = ctx.upload(hg)
@@ -276,7 +276,7 @@ def c2_section(ctx, limits_ms):
         dt = time.perf_counter() - t
         ctx.set_profiling(False)
         done = r.status == 0
-        print(f"# C2 {name}: {'exact' if done else 'lower bound'} {r.count} in {dt:.2f}s", file=sys.stderr,
+        print(f"# rmat16 {name}: {'exact' if done else 'lower bound'} {r.count} in {dt:.2f}s", file=sys.stderr,
               flush=True)
         tot = kernel_totals([[r]])
         entry = {"name": name, "count": r.count, "exact": done, "seconds": round(dt, 3),
@@ -293,7 +293,7 @@ def c2_section(ctx, limits_ms):
             entry["lds_lists_sample"] = {"seconds": 5, "default_items_per_s": round(rate[False], 1),
                                          "lds_lists_items_per_s": round(rate[True], 1),
                                          "ratio": round(rate[True] / rate[False], 3)}
-            print(f"# C2 {name}: lds_lists {rate[True]:.3g} items/s, default {rate[False]:.3g}", file=sys.stderr,
+            print(f"# rmat16 {name}: lds_lists {rate[True]:.3g} items/s, default {rate[False]:.3g}", file=sys.stderr,
                   flush=True)
         out.append(entry)
         q.free()
@@ -305,7 +305,7 @@ def c2_section(ctx, limits_ms):
 
 
 def batched_section(ctx, nqueries, cap, item_budget, sample=0):
-    """The C3-shaped query set over all its queries, in the reference's
+    """The R-MAT scale 20 query set over all its queries, in the reference's
     matching order: the plain path query by query, then one batch call, both
     with the same per-query item budget (a query over it stops with a lower
     bound on both paths); queries/s over every query, and every count that
@@ -650,9 +650,9 @@ def main():
     ap.add_argument("--group", type=int, default=16, help="headline queries per step (one batch call)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--c2-limits-ms", type=str, default="5000,45000",
+    ap.add_argument("--rmat16-limits-ms", type=str, default="5000,45000",
                     help="time limits of the two uncapped R-MAT 16 queries (tree, cycles+3)")
-    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-rmat16", action="store_true")
     ap.add_argument("--batched-queries", type=int, default=256)
     ap.add_argument("--batched-cap", type=int, default=100_000_000)
     ap.add_argument("--batched-item-budget", type=int, default=20_000_000_000)
@@ -667,12 +667,12 @@ def main():
                     help="comma-separated further arrival rates for the served workload ('' for none)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="profiling: the headline steps only")
-    ap.add_argument("--sections", type=str, default="c4,cpu,c2,batched,served",
-                    help="comma-separated sections to run (the headline c4 always runs unless omitted here)")
+    ap.add_argument("--sections", type=str, default="headline,cpu,rmat16,batched,served",
+                    help="comma-separated sections to run (the headline always runs unless omitted here)")
     args = ap.parse_args()
     sections = set(x.strip() for x in args.sections.split(","))
     if args.headline_only:
-        sections = {"c4"}
+        sections = {"headline"}
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; multi-GPU sharding is a design only "
                                    "(queries or root candidates split across GPUs, one u64 count reduction)"}))
@@ -681,10 +681,10 @@ def main():
     ctx = csm.Context(0, int(args.arena_gb * (1 << 30)))
     ms_copy = ctx.copy_probe(1 << 30, 10)
     copy_gbs = 2.0 * (1 << 30) / (ms_copy * 1e6)
-    if "c4" not in sections:  # partial runs for profiling and development
+    if "headline" not in sections:  # partial runs for profiling and development
         line = {"metric": METRIC, "partial": sorted(sections)}
-        if "c2" in sections:
-            line["c2_full"] = c2_section(ctx, [int(x) for x in args.c2_limits_ms.split(",")])
+        if "rmat16" in sections:
+            line["rmat16_full"] = rmat16_section(ctx, [int(x) for x in args.rmat16_limits_ms.split(",")])
         if "batched" in sections:
             line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_item_budget,
                                               args.batched_sample)
@@ -695,12 +695,12 @@ def main():
         ctx.close()
         print(json.dumps(line), flush=True)
         return
-    c4 = c4_section(ctx, args)
-    elapsed, tot = c4["elapsed"], c4["tot"]
-    nv, ne, maxdeg = c4["graph"]
+    hl = headline_section(ctx, args)
+    elapsed, tot = hl["elapsed"], hl["tot"]
+    nv, ne, maxdeg = hl["graph"]
     line = {
         "metric": METRIC,
-        "value": round(c4["embeddings"] / elapsed, 1),
+        "value": round(hl["embeddings"] / elapsed, 1),
         "unit": "embeddings/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -713,30 +713,30 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"batched path, Chung-Lu power-law graph (16M vertices, 256M edge samples, exponent 2.1, "
-                        f"max degree 50,000, 64 Zipf(0.8) labels), {C4_QUERIES} random-walk queries of 12..24 "
+                        f"max degree 50,000, 64 Zipf(0.8) labels), {HEADLINE_QUERIES} random-walk queries of 12..24 "
                         f"vertices with 2..6 non-tree edges, count-only, cap 1e9 per query; one step = one "
                         f"csm_batch_match over {args.group} of the queries (groups in turn)",
             "graph_vertices": nv,
             "graph_edges": ne,
             "max_degree": maxdeg,
-            "queries": C4_QUERIES,
+            "queries": HEADLINE_QUERIES,
             "queries_per_step": args.group,
-            "cap_per_query": C4_CAP,
+            "cap_per_query": HEADLINE_CAP,
             "parallelism": "single GPU",
-            **c4["detail"],
+            **hl["detail"],
         },
         "roofline": roofline_from(tot, "batch_join_chung_lu_16m.md", cls=csm.K_BATCH),
         "kernels": kernels_json(tot),
         "copy_probe_GBps": round(copy_gbs, 1),
     }
-    hg, hqs = c4["hg"], c4["hqs"]
-    del c4
+    hg, hqs = hl["hg"], hl["hqs"]
+    del hl
     if "cpu" in sections and not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(hg, hqs[:4], C4_CAP, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(hg, hqs[:4], HEADLINE_CAP, args.cpu_seconds)
     del hg, hqs
-    if "c2" in sections and not args.no_c2:
-        lim = [int(x) for x in args.c2_limits_ms.split(",")]
-        line["c2_full"] = c2_section(ctx, lim)
+    if "rmat16" in sections and not args.no_rmat16:
+        lim = [int(x) for x in args.rmat16_limits_ms.split(",")]
+        line["rmat16_full"] = rmat16_section(ctx, lim)
     if "batched" in sections and not args.no_batched:
         line["batched"] = batched_section(ctx, args.batched_queries, args.batched_cap, args.batched_item_budget,
                                           args.batched_sample)

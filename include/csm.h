@@ -289,8 +289,7 @@ csm_status csm_match_noalloc(csm_ctx *ctx, const csm_graph *g, csm_query *q,
 csm_status csm_query_reorder(csm_ctx *ctx, const csm_graph *g, csm_query *q, int refine);
 
 /* Symmetry-breaking conditions for counting, no counterpart in the reference
- * (which counts every automorphic image; SURVEY's next-step list names it,
- * behind an option): the labelled automorphism group Aut(Q) walked as a
+ * (which counts every automorphic image), behind an option: the labelled automorphism group Aut(Q) walked as a
  * stabiliser chain, each vertex's orbit under the automorphisms fixing the
  * vertices before it giving f(v) < f(w) for the orbit's other members
  * (Grochow & Kellis, RECOMB 2007).  Exactly one embedding of every Aut(Q)

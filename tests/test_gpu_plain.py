@@ -205,7 +205,7 @@ def test_caps_match_oracle(ctx):
     assert done > 5
 
 
-def test_config1_er_full_list(ctx):
+def test_er_1000_4vertex_full_list(ctx):
     """ER 1000/4000 (seed 1), 4 uniform labels, 4-vertex random-walk query (seed 2), full list."""
     hg = csm.gen_erdos_renyi(1000, 4000, 4, 1)
     hq = csm.gen_query(hg, 4, csm.QUERY_INDUCED, 2)
@@ -215,14 +215,14 @@ def test_config1_er_full_list(ctx):
     g = ctx.upload(hg)
     q = g.prepare(hq.plan())
     res, rows = q.match(want_list=True, list_capacity=max(oc, 1))
-    print("config1 count", res.count, flush=True)
+    print("er 1000 count", res.count, flush=True)
     assert res.count == oc
     assert np.array_equal(rows, np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, 4))
     q.free()
     g.free()
 
 
-def test_config2_rmat16_counts(ctx):
+def test_rmat16_8vertex_first_100k_vs_oracle(ctx):
     """R-MAT scale 16, 16 Zipf labels: two 8-vertex queries, tree-like and with
     3 extra cycle edges.  Both have more than 1e10 embeddings, so the first
     100,000 in the matching order (listed, sorted) must be the oracle's, with
@@ -242,7 +242,7 @@ def test_config2_rmat16_counts(ctx):
                                            refine=False)
         q = g.prepare(hq.plan())
         res, rows = q.match(cap=n_list, want_list=True)
-        print("config2", mode, res.count, bool(res.capped), f"oracle {secs:.1f}s", flush=True)
+        print("rmat16 8-vertex", mode, res.count, bool(res.capped), f"oracle {secs:.1f}s", flush=True)
         assert (res.count, bool(res.capped)) == (oc, ocap) == (n_list, True)
         want = np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, len(ql))
         assert np.array_equal(rows, want)

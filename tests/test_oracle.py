@@ -186,8 +186,9 @@ def test_refinement_equals_spec():
 
 
 def test_refinement_spec_on_the_worked_example():
-    """SURVEY App. B's refined sets (u0,u2: {0,4,5,6,7,8}; u1: {1,6,8};
-    u3: {2,9}; u4: {3,10}) from the spec over the traced clusters."""
+    """The refined sets of the reference's worked example (example_q_01 over
+    example_g_01; u0,u2: {0,4,5,6,7,8}; u1: {1,6,8}; u3: {2,9}; u4: {3,10}),
+    traced by hand through its clusters, from the spec."""
     qp, gp = example_paths()
     q = O.Query.read(qp)
     d = O.Data.read(gp)
