@@ -12,10 +12,13 @@ embeddings counted in the timed steps / their time.  The same 128 queries
 are also run once per query on the plain path (csm_match) and once in a
 single batch call, for queries/s and every count compared.  The roofline
 entry is the persistent join kernel, timed with HIP events on its stream:
-algorithmic bytes = the reference's extension pass over the same rows
-(join.cu:99-177: row, offsets, and id + candidate word per adjacency entry
-of the row's parent vertex, rows written), with the kernel's own logical
-bytes beside it.
+algorithmic bytes = the kernel's own logical reads and writes, counted by
+the kernel (rows read, list locators, list entries examined and searched,
+hub-bitmap words, rows written).  The plain path's join classes also carry
+the reference's extension bytes for the same rows (join.cu:99-177: row,
+offsets, and id + candidate word per adjacency entry of the row's parent
+vertex, rows written) as `ref_bytes_per_launch`; the batch kernel does not
+read its rows' parent degrees, so its entry has null there.
 
 rmat16_full: the plain path on R-MAT scale 16 (65,536 vertices, 2^20 edge
 samples, a=0.57 b=0.19 c=0.19), 16 labels Zipf(1.0), two 8-vertex
@@ -127,14 +130,14 @@ def roofline_from(tot, profile, cls=None):
         "launches": launches,
         "avg_launch_us": round(ns / launches / 1e3, 2) if launches else None,
         "bytes_per_launch": byts // launches if launches else None,
-        "ref_bytes_per_launch": ref // launches if launches else None,
-        "ref_equivalent_GBps": round(ref / ns, 1) if ns else None,
+        "ref_bytes_per_launch": ref // launches if launches and ref else None,
+        "ref_equivalent_GBps": round(ref / ns, 1) if ns and ref else None,
     }
 
 
 def kernels_json(tot):
     return {csm.KERNEL_NAMES[c]: {"launches": v[0], "ms": round(v[1] / 1e6, 3), "GB": round(v[2] / 1e9, 3),
-                                  "ref_GB": round(v[3] / 1e9, 3)}
+                                  "ref_GB": round(v[3] / 1e9, 3) if v[3] else None}
             for c, v in tot.items() if v[0]}
 
 
@@ -404,7 +407,7 @@ def batched_section(ctx, nqueries, cap, item_budget, sample=0):
         "kernels": kernels_json(tot),
         "join_kernel": {"ms": round(bk[1] / 1e6, 1), "algorithmic_GB": round(bk[2] / 1e9, 2),
                         "GB_per_s": round(bk[2] / bk[1], 1) if bk[1] else None,
-                        "ref_GB": round(bk[3] / 1e9, 2)},
+                        "ref_GB": round(bk[3] / 1e9, 2) if bk[3] else None},
         # candidate tests (row x driver entry) done by each path; capped
         # queries overshoot their cap by different amounts on the two paths
         "items": {"batch": sum(x.items for x in res), "plain": sum(p.items for p in plain),
