@@ -71,3 +71,31 @@ def list_digest(rows):
     for t in rows:
         h.update((" ".join(map(str, t)) + "\n").encode())
     return h.hexdigest()
+
+
+def _cycle(n):
+    return [(i, (i + 1) % n) for i in range(n)]
+
+
+def max_size_cases():
+    """(name, data labels, data edges, query labels, query edges, count) for
+    32-vertex queries on labelled cycles, the counts by closed form:
+    * a path labelled i mod 4 in a 64-cycle labelled j mod 4: only the
+      forward walk keeps the label order, one per start labelled 0: 16;
+    * the same path closed into a 32-cycle, in the 64-cycle: vertices 31
+      steps apart are never adjacent there: 0;
+    * that 32-cycle in a 32-cycle labelled j mod 4: the rotations by a
+      multiple of 4: 8;
+    * one label everywhere, the 32-cycle in the 32-cycle: the 64 dihedral
+      images; the 32-vertex path in a 40-cycle: 40 starts x 2 directions: 80;
+    * a query with a label the data graph does not hold: 0."""
+    path = [(i, i + 1) for i in range(31)]
+    lab4 = [i % 4 for i in range(32)]
+    return [
+        ("path in C64, 4 labels", [j % 4 for j in range(64)], _cycle(64), lab4, path, 16),
+        ("C32 in C64, 4 labels", [j % 4 for j in range(64)], _cycle(64), lab4, _cycle(32), 0),
+        ("C32 in C32, 4 labels", [j % 4 for j in range(32)], _cycle(32), lab4, _cycle(32), 8),
+        ("C32 in C32, 1 label", [0] * 32, _cycle(32), [0] * 32, _cycle(32), 64),
+        ("path in C40, 1 label", [0] * 40, _cycle(40), [0] * 32, path, 80),
+        ("absent label", [j % 4 for j in range(64)], _cycle(64), lab4[:31] + [7], path, 0),
+    ]

@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 import clustersm_amd as csm
-from helpers import brute_force, example_paths, list_digest, load_example, load_golden, rand_graph, rand_query
+from helpers import (brute_force, example_paths, list_digest, load_example, load_golden, max_size_cases, rand_graph,
+                     rand_query)
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -399,3 +400,29 @@ def test_symmetry_breaking_counts_vs_oracle(ctx):
         g.free()
     print("symmetric queries with embeddings:", symmetric, flush=True)
     assert symmetric >= 5
+
+
+
+def test_max_size_queries_closed_forms(ctx):
+    """32-vertex queries (the largest a plan takes) on labelled cycles, whose
+    counts follow from the cycles' rotations and reflections (listed in
+    helpers.max_size_cases), on the plain path (count and list) and the
+    batched path, each also against the oracle; the one-label 32-cycle has
+    64 automorphisms, so the symmetry option counts 1."""
+    for name, dl, de, ql, qe, want in max_size_cases():
+        print("case", name, flush=True)
+        oq = O.Query.from_arrays(ql, qe)
+        oc, ocap, orows, _, _ = O.match(O.Data.from_arrays(dl, de), oq, oq.plan(), want_list=True)
+        assert (oc, bool(ocap)) == (want, False), name
+        g = ctx.upload_edges(dl, de)
+        q = g.prepare(csm.HostGraph.from_edges(ql, qe).plan())
+        res, rows = q.match(want_list=True, list_capacity=max(want, 1))
+        assert (res.count, bool(res.capped), res.status) == (want, False, 0), name
+        assert np.array_equal(rows, np.array(sorted(map(tuple, orows.tolist())), np.uint32).reshape(-1, 32)), name
+        (b,) = csm.batch_match(g, [q])
+        assert (b.count, bool(b.capped), b.status) == (want, False, 0), name
+        if name == "C32 in C32, 1 label":
+            s, _ = q.match(symmetry=True)
+            assert (s.count, s.automorphisms) == (1, 64), name
+        q.free()
+        g.free()

@@ -7,7 +7,8 @@ import random
 import numpy as np
 import pytest
 
-from helpers import brute_force, example_paths, load_example, load_golden, rand_graph, rand_query, list_digest
+from helpers import (brute_force, example_paths, load_example, load_golden, max_size_cases, rand_graph, rand_query,
+                     list_digest)
 from oracle import oracle as O
 
 
@@ -257,3 +258,21 @@ def test_oracle_rejects_malformed(tmp_path):
         O.Query.read(str(bad))
     with pytest.raises(O.OracleError):  # disconnected query
         O.Query.from_arrays([0, 0, 0, 0], [(0, 1), (2, 3)]).plan()
+
+
+@pytest.mark.parametrize("case", max_size_cases(), ids=lambda c: c[0])
+def test_max_size_queries_closed_forms(case):
+    """32-vertex queries on labelled cycles: the oracle's count and list
+    against counts fixed by the cycles' rotations and reflections; every
+    listed row is an embedding (labels kept, edges mapped to edges,
+    injective)."""
+    name, dl, de, ql, qe, want = case
+    oq = O.Query.from_arrays(ql, qe)
+    c, capped, rows, _, _ = O.match(O.Data.from_arrays(dl, de), oq, oq.plan(), want_list=True)
+    assert (c, bool(capped)) == (want, False)
+    adj = {(a, b) for a, b in de} | {(b, a) for a, b in de}
+    assert len({tuple(r) for r in rows.tolist()}) == want
+    for r in rows.tolist():
+        assert len(set(r)) == 32
+        assert all(dl[r[i]] == ql[i] for i in range(32))
+        assert all((r[a], r[b]) in adj for a, b in qe)
